@@ -1,0 +1,838 @@
+// sb_api.cpp -- C ABI of the solver: host-only helpers, context, problems and
+// the single-instance solve loop (the reference's SB/CUDA_GPU/m2_opt.cu
+// main(), lines 100-220, re-designed around one fused launch per step).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sb_host.h"
+
+using namespace sbk;
+
+// ---------------------------------------------------------------------------
+// Errors.
+
+namespace sbh {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? SB_E_NOMEM : SB_E_HIP;
+}
+
+}  // namespace sbh
+
+#define HIP_TRY(call)                                   \
+    do {                                                \
+        hipError_t _e = (call);                         \
+        if (_e != hipSuccess) return sbh::hip_fail(_e, #call); \
+    } while (0)
+
+extern "C" SB_API const char *sb_last_error_message(void) { return sbh::g_last_error.c_str(); }
+
+extern "C" SB_API const char *sb_version(void) { return "annealing_amd-sb 0.1.0 (gfx950)"; }
+
+extern "C" SB_API const char *sb_status_string(int status)
+{
+    switch (status) {
+    case SB_OK: return "ok";
+    case SB_E_INVALID: return "invalid argument";
+    case SB_E_IO: return "i/o error";
+    case SB_E_PARSE: return "malformed edge list";
+    case SB_E_RANGE: return "spin index out of range";
+    case SB_E_INEXACT: return "J not exactly representable in the requested format";
+    case SB_E_NOMEM: return "out of memory";
+    case SB_E_HIP: return "HIP runtime error";
+    case SB_E_BUDGET: return "request exceeds the HBM budget";
+    case SB_E_NODEVICE: return "no usable GPU";
+    case SB_E_TIMEOUT: return "timed out";
+    case SB_E_CANCELLED: return "cancelled";
+    case SB_E_NOTFOUND: return "unknown ticket";
+    case SB_E_SHUTDOWN: return "server shutting down";
+    default: return "unknown status";
+    }
+}
+
+extern "C" SB_API int sb_default_params(sb_params *p)
+{
+    if (!p) return SB_E_INVALID;
+    std::memset(p, 0, sizeof(*p));
+    // SB/CUDA_GPU/m2_opt.cu:10-22
+    p->variant = SB_ASB;
+    p->precision = SB_PREC_FP32;
+    p->init = SB_INIT_REFERENCE;
+    p->schedule = SB_SCHED_LINEAR;
+    p->steps = 100;   // STEP
+    p->replicas = 1;  // TIMES
+    p->M = 2;         // M
+    p->dt = 1.0;      // DeltaT (local sub-step deltaT = DeltaT / M = 0.5)
+    p->a0 = 1.0;      // detune
+    p->c0 = 0.0;      // xi: auto, 0.7 a0 / (sigma sqrt(N)) = 0.01565 for K2000 +-1
+    p->K = 1.0;       // K
+    p->seed = 0;
+    return SB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-only instance helpers.
+
+namespace {
+
+struct Cursor {
+    const char *p, *end;
+};
+
+void skip_ws(Cursor &c)
+{
+    while (c.p < c.end && (*c.p == ' ' || *c.p == '\t' || *c.p == '\n' || *c.p == '\r' || *c.p == '\v' ||
+                           *c.p == '\f'))
+        ++c.p;
+}
+
+// 0 = end of input, 1 = token read, -1 = malformed token.
+int next_token(Cursor &c, char *buf, size_t cap)
+{
+    skip_ws(c);
+    if (c.p >= c.end) return 0;
+    size_t n = 0;
+    while (c.p < c.end && !(*c.p == ' ' || *c.p == '\t' || *c.p == '\n' || *c.p == '\r' || *c.p == '\v' ||
+                            *c.p == '\f')) {
+        if (n + 1 >= cap) return -1;
+        buf[n++] = *c.p++;
+    }
+    buf[n] = 0;
+    return 1;
+}
+
+int read_index(Cursor &c, long long *v)
+{
+    char buf[40];
+    int t = next_token(c, buf, sizeof(buf));
+    if (t <= 0) return t;
+    char *e = nullptr;
+    long long x = std::strtoll(buf, &e, 10);
+    if (e == buf || *e) return -1;
+    *v = x;
+    return 1;
+}
+
+int read_weight(Cursor &c, double *v)
+{
+    char buf[80];
+    int t = next_token(c, buf, sizeof(buf));
+    if (t <= 0) return t;
+    char *e = nullptr;
+    double x = std::strtod(buf, &e);
+    if (e == buf || *e || !std::isfinite(x)) return -1;
+    *v = x;
+    return 1;
+}
+
+}  // namespace
+
+// SB/CUDA_GPU/m2_opt.cu:111-123: "%d" for N, then "%d%d%d" triples to EOF,
+// couplings[a*N+b] = couplings[b*N+a] = w; assert(a != b).
+extern "C" SB_API int sb_edgelist_parse(const char *text, size_t len, int one_based, int32_t *N_out,
+                                        float *J, int32_t ldj)
+{
+    if (!text || !N_out) return SB_E_INVALID;
+    Cursor c{text, text + len};
+    long long n = 0;
+    if (read_index(c, &n) != 1 || n < 1 || n > (1ll << 20)) {
+        sbh::set_error("edge list: missing or invalid spin count");
+        return SB_E_PARSE;
+    }
+    *N_out = (int32_t)n;
+    if (!J) return SB_OK;
+    if (ldj < n) return SB_E_INVALID;
+    for (long long i = 0; i < n; ++i) std::memset(J + (size_t)i * ldj, 0, sizeof(float) * (size_t)n);
+    long long line = 1;
+    for (;;) {
+        long long a, b;
+        double w;
+        int ra = read_index(c, &a);
+        if (ra == 0) break;
+        ++line;
+        if (ra < 0 || read_index(c, &b) != 1 || read_weight(c, &w) != 1) {
+            sbh::set_error("edge list: malformed triple #" + std::to_string(line - 1));
+            return SB_E_PARSE;
+        }
+        if (one_based) {
+            a -= 1;
+            b -= 1;
+        }
+        if (a < 0 || b < 0 || a >= n || b >= n) {
+            sbh::set_error("edge list: index out of range in triple #" + std::to_string(line - 1));
+            return SB_E_RANGE;
+        }
+        if (a == b) {
+            sbh::set_error("edge list: self-coupling i == j in triple #" + std::to_string(line - 1));
+            return SB_E_INVALID;
+        }
+        J[(size_t)a * ldj + b] = (float)w;
+        J[(size_t)b * ldj + a] = (float)w;
+    }
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_instance_stats(int32_t N, const float *J, int32_t ldj, double *sigma,
+                                        int32_t *is_integer, double *max_abs)
+{
+    if (N < 1 || !J || ldj < N) return SB_E_INVALID;
+    double s2 = 0.0, mx = 0.0;
+    int integer = 1;
+    for (int i = 0; i < N; ++i) {
+        const float *row = J + (size_t)i * ldj;
+        for (int j = 0; j < N; ++j) {
+            double v = row[j];
+            if (i == j) {
+                if (v != 0.0) return SB_E_INVALID;
+                continue;
+            }
+            if (!std::isfinite(v)) return SB_E_INVALID;
+            s2 += v * v;
+            mx = std::max(mx, std::fabs(v));
+            if (v != std::floor(v)) integer = 0;
+        }
+    }
+    if (sigma) *sigma = N > 1 ? std::sqrt(s2 / ((double)N * (double)(N - 1))) : 0.0;
+    if (is_integer) *is_integer = integer;
+    if (max_abs) *max_abs = mx;
+    return SB_OK;
+}
+
+extern "C" SB_API double sb_c0_auto(int32_t variant, int32_t N, double a0, double sigma)
+{
+    if (N < 1 || !(sigma > 0.0)) return 0.0;
+    if (variant == SB_ASB) return 0.7 * a0 / (sigma * std::sqrt((double)N));
+    return 0.5 / (std::sqrt((double)N) * sigma);
+}
+
+namespace {
+
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1p-53; }
+    double normal()
+    {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 0x1p-60) u1 = 0x1p-60;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+}  // namespace
+
+extern "C" SB_API int sb_generate(int32_t kind, int32_t N, double p, uint64_t seed, float *J)
+{
+    if (N < 1 || !J || kind < 0 || kind > 3) return SB_E_INVALID;
+    if ((kind == 0 || kind == 3) && !(p >= 0.0 && p <= 1.0)) return SB_E_INVALID;
+    SplitMix rng{seed};
+    const double inv_sqrt_n = 1.0 / std::sqrt((double)N);
+    for (int i = 0; i < N; ++i) J[(size_t)i * N + i] = 0.0f;
+    for (int i = 0; i < N; ++i) {
+        for (int j = i + 1; j < N; ++j) {
+            float v = 0.0f;
+            switch (kind) {
+            case 0: {  // Erdos-Renyi, w = +-1, J = -W
+                double u = rng.uniform();
+                uint64_t b = rng.next();
+                if (u < p) v = (b & 1) ? -1.0f : 1.0f;
+                break;
+            }
+            case 1:  // SK
+                v = (float)(rng.normal() * inv_sqrt_n);
+                break;
+            case 2:  // complete graph +-1
+                v = (rng.next() & 1) ? 1.0f : -1.0f;
+                break;
+            case 3: {  // diluted SK
+                double u = rng.uniform();
+                double z = rng.normal();
+                if (u < p) v = (float)(z * inv_sqrt_n);
+                break;
+            }
+            }
+            J[(size_t)i * N + j] = v;
+            J[(size_t)j * N + i] = v;
+        }
+    }
+    return SB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Coefficients.
+
+extern "C" SB_API int sb_step_coeffs_for(const sb_params *p, double c0, int32_t k, sb_step_coeffs *out)
+{
+    if (!p || !out || p->steps < 1 || k < 0 || k >= p->steps) return SB_E_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->variant = p->variant;
+    out->M = p->M;
+    out->a0 = p->a0;
+    out->K = p->K;
+    out->dts = p->M > 0 ? p->dt / (double)p->M : 0.0;
+    if (p->variant == SB_ASB) {
+        out->beta = (float)(p->dt * c0);  // m2_opt.cu:148 coeff1 = DeltaT * xi
+        out->do_local = (k + 1 < p->steps) ? 1 : 0;
+        out->p_next = (k + 1 < p->steps) ? (float)(k + 1) / (float)p->steps : 0.0f;  // m2_opt.cu:158
+    } else {
+        double pk = (double)k / (double)p->steps;
+        double a = p->a0 * pk;
+        out->alpha = (float)(p->dt * (p->a0 - a));
+        out->beta = (float)(p->dt * c0);
+        out->gamma = (float)(p->dt * p->a0);
+        out->p_next = (float)pk;
+    }
+    return SB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Context.
+
+extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
+{
+    if (!out) return SB_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        sbh::set_error("no HIP device visible");
+        return SB_E_NODEVICE;
+    }
+    if (device < 0 || device >= n) return SB_E_INVALID;
+    HIP_TRY(hipSetDevice(device));
+    sb_ctx *c = new sb_ctx();
+    c->device = device;
+    // Default stream priority only.
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        delete c;
+        return sbh::hip_fail(e, "sb_ctx_create");
+    }
+    const char *at = std::getenv("SB_AUTOTUNE");
+    if (at && at[0] == '0') c->autotune = false;
+    *out = c;
+    return SB_OK;
+}
+
+static void free_ws(sb_ctx *c)
+{
+    hipFree(c->X0);
+    hipFree(c->X1);
+    hipFree(c->Y);
+    hipFree(c->H);
+    hipFree(c->part);
+    hipFree(c->S);
+    hipFree(c->tmp);
+    c->X0 = c->X1 = c->Y = nullptr;
+    c->H = c->part = nullptr;
+    c->S = nullptr;
+    c->tmp = nullptr;
+    c->state_cap = c->H_cap = c->part_cap = c->S_cap = c->tmp_cap = 0;
+}
+
+extern "C" SB_API int sb_ctx_destroy(sb_ctx *ctx)
+{
+    if (!ctx) return SB_E_INVALID;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    free_ws(ctx);
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SB_OK;
+}
+
+namespace sbh {
+
+template <typename T>
+static int grow(T **p, size_t *cap, size_t need)
+{
+    if (*cap >= need) return SB_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc((void **)p, need * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, "hipMalloc(workspace)");
+    }
+    *cap = need;
+    return SB_OK;
+}
+
+int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N)
+{
+    int rc;
+    if (ctx->state_cap < state_elems) {
+        hipStreamSynchronize(ctx->stream);
+        size_t c0 = ctx->state_cap, c1 = ctx->state_cap, c2 = ctx->state_cap;
+        if ((rc = grow(&ctx->X0, &c0, state_elems))) return rc;
+        if ((rc = grow(&ctx->X1, &c1, state_elems))) return rc;
+        if ((rc = grow(&ctx->Y, &c2, state_elems))) return rc;
+        ctx->state_cap = state_elems;
+    }
+    if ((rc = grow(&ctx->H, &ctx->H_cap, (size_t)std::max(R, 1)))) return rc;
+    if ((rc = grow(&ctx->part, &ctx->part_cap, std::max<size_t>(energy_partial_elems(N, R), 1)))) return rc;
+    if ((rc = grow(&ctx->S, &ctx->S_cap, std::max<size_t>((size_t)R * N, 1)))) return rc;
+    if ((rc = grow(&ctx->tmp, &ctx->tmp_cap, std::max<size_t>((size_t)R * N, 1)))) return rc;
+    return SB_OK;
+}
+
+}  // namespace sbh
+
+// ---------------------------------------------------------------------------
+// Problems.
+
+extern "C" SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, int32_t ldj, sb_problem **out)
+{
+    if (!ctx || !J || !out || N < 1 || ldj < N) return SB_E_INVALID;
+    *out = nullptr;
+    double sigma = 0, mx = 0;
+    int32_t integer = 0;
+    for (int i = 0; i < N; ++i) {
+        const float *row = J + (size_t)i * ldj;
+        for (int j = i + 1; j < N; ++j)
+            if (row[j] != J[(size_t)j * ldj + i]) {
+                sbh::set_error("J is not symmetric");
+                return SB_E_INVALID;
+            }
+    }
+    int rc = sb_instance_stats(N, J, ldj, &sigma, &integer, &mx);
+    if (rc) {
+        sbh::set_error("J has a nonzero diagonal or a non-finite entry");
+        return rc;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    sb_problem *p = new sb_problem();
+    p->ctx = ctx;
+    p->N = N;
+    p->ld = round_up(N, kSpinPad);
+    p->integer = integer != 0;
+    p->sigma = sigma;
+    p->max_abs = mx;
+    const size_t elems = (size_t)N * p->ld;
+    std::vector<float> host(elems, 0.0f);
+    for (int i = 0; i < N; ++i) std::memcpy(&host[(size_t)i * p->ld], J + (size_t)i * ldj, sizeof(float) * N);
+    hipError_t e = hipMalloc((void **)&p->J32, elems * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(p->J32, host.data(), elems * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->Jbf16, elems * sizeof(uint16_t));
+    if (e == hipSuccess) e = launch_pack_bf16(p->J32, elems, p->Jbf16, ctx->stream);
+    if (e == hipSuccess && p->integer && mx <= 16.0) {
+        e = hipMalloc((void **)&p->Jfp8, elems);
+        if (e == hipSuccess) e = launch_pack_fp8(p->J32, elems, p->Jfp8, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(p->J32);
+        hipFree(p->Jbf16);
+        hipFree(p->Jfp8);
+        delete p;
+        return sbh::hip_fail(e, "sb_problem_create");
+    }
+    *out = p;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *path, int one_based,
+                                                    sb_problem **out)
+{
+    if (!ctx || !path || !out) return SB_E_INVALID;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) {
+        sbh::set_error(std::string("cannot open ") + path);
+        return SB_E_IO;
+    }
+    std::string text;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) return SB_E_IO;
+    int32_t N = 0;
+    int rc = sb_edgelist_parse(text.data(), text.size(), one_based, &N, nullptr, 0);
+    if (rc) return rc;
+    std::vector<float> J((size_t)N * N, 0.0f);
+    rc = sb_edgelist_parse(text.data(), text.size(), one_based, &N, J.data(), N);
+    if (rc) return rc;
+    return sb_problem_create(ctx, N, J.data(), N, out);
+}
+
+extern "C" SB_API int sb_problem_destroy(sb_problem *p)
+{
+    if (!p) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->stream);
+    hipFree(p->J32);
+    hipFree(p->Jbf16);
+    hipFree(p->Jfp8);
+    delete p;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_info(const sb_problem *p, int32_t *N, int32_t *is_integer, double *sigma,
+                                      int32_t *split_jfmt)
+{
+    if (!p) return SB_E_INVALID;
+    if (N) *N = p->N;
+    if (is_integer) *is_integer = p->integer ? 1 : 0;
+    if (sigma) *sigma = p->sigma;
+    if (split_jfmt) *split_jfmt = p->Jfp8 ? SB_J_FP8 : SB_J_BF16;
+    return SB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel choice and autotuning.
+
+namespace sbh {
+
+int check_params(const sb_params *p, int N, double sigma, double *c0_out)
+{
+    if (!p) return SB_E_INVALID;
+    if (p->variant < SB_ASB || p->variant > SB_DSB) return SB_E_INVALID;
+    if (p->precision < SB_PREC_FP32 || p->precision > SB_PREC_ORDERED) return SB_E_INVALID;
+    if (p->init != SB_INIT_UNIFORM && p->init != SB_INIT_REFERENCE) return SB_E_INVALID;
+    if (p->schedule != SB_SCHED_LINEAR) return SB_E_INVALID;
+    if (p->steps < 1 || p->replicas < 1) return SB_E_INVALID;
+    if (p->variant == SB_ASB && p->M < 1) return SB_E_INVALID;
+    if (!std::isfinite(p->dt) || !std::isfinite(p->a0) || !std::isfinite(p->c0) || !std::isfinite(p->K))
+        return SB_E_INVALID;
+    double c0 = p->c0 > 0.0 ? p->c0 : sb_c0_auto(p->variant, N, p->a0, sigma);
+    if (c0_out) *c0_out = c0;
+    return SB_OK;
+}
+
+std::vector<int> candidate_families(int precision, int variant, const sb_problem *prob)
+{
+    if (precision == SB_PREC_FP32) return {FAM_FP32};
+    if (precision == SB_PREC_SPLIT) {
+        if (prob->Jfp8) {
+            if (variant == SB_DSB) return {FAM_DSB_FP8, FAM_SPLIT_FP8, FAM_SPLIT_BF16};
+            return {FAM_SPLIT_BF16, FAM_SPLIT_FP8};
+        }
+        return {FAM_SPLIT_BF16};
+    }
+    return {};
+}
+
+const void *family_J(int family, const sb_problem *prob)
+{
+    switch (family) {
+    case FAM_FP32: return prob->J32;
+    case FAM_SPLIT_BF16: return prob->Jbf16;
+    case FAM_SPLIT_FP8:
+    case FAM_DSB_FP8: return prob->Jfp8;
+    default: return nullptr;
+    }
+}
+
+static int bucket(int v)
+{
+    int b = 1;
+    while (b < v) b <<= 1;
+    return b;
+}
+
+// Heuristic shape when not tuning: the largest tile that still gives at least
+// one workgroup per CU, else the smallest tile.
+static int heuristic_shape(int family, int N, int R)
+{
+    int best = -1;
+    long best_area = -1;
+    for (int s = 0; s < shape_count(family); ++s) {
+        TileShape sh = shape_at(family, s);
+        long tiles = (long)((N + sh.mt() - 1) / sh.mt()) * ((R + sh.nt() - 1) / sh.nt());
+        long area = (long)sh.mt() * sh.nt();
+        if (tiles >= 256 && area > best_area) {
+            best = s;
+            best_area = area;
+        }
+    }
+    if (best >= 0) return best;
+    long smallest = 1L << 40;
+    for (int s = 0; s < shape_count(family); ++s) {
+        TileShape sh = shape_at(family, s);
+        long area = (long)sh.mt() * sh.nt();
+        if (area < smallest) {
+            smallest = area;
+            best = s;
+        }
+    }
+    return best;
+}
+
+int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int variant, int R, int *family,
+                  int *shape)
+{
+    std::vector<int> fams = candidate_families(precision, variant, prob);
+    if (fams.empty()) return SB_E_INVALID;
+    const int N = prob->N;
+    auto key = std::make_tuple(fams[0] * 8 + precision, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
+    auto it = ctx->tune.find(key);
+    if (it != ctx->tune.end()) {
+        *family = it->second.first;
+        *shape = it->second.second;
+        return SB_OK;
+    }
+    if (!ctx->autotune) {
+        *family = fams[0];
+        *shape = heuristic_shape(fams[0], N, R);
+        ctx->tune[key] = {*family, *shape};
+        return SB_OK;
+    }
+    // Time each candidate for a few launches on the workspace (its contents
+    // are overwritten by the init that follows).
+    int rc = ensure_workspace(ctx, (size_t)R * prob->ld, R, N);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->X0, 0, (size_t)R * prob->ld * sizeof(float), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->Y, 0, (size_t)R * prob->ld * sizeof(float), ctx->stream));
+    StepArgs a{};
+    a.X = ctx->X0;
+    a.Xn = ctx->X1;
+    a.Y = ctx->Y;
+    a.ldj = prob->ld;
+    a.ldx = prob->ld;
+    a.N = N;
+    a.R = R;
+    a.c.variant = variant;
+    a.c.M = 1;
+    float best_ms = 1e30f;
+    int best_f = fams[0], best_s = 0;
+    const int reps = 3;
+    for (int f : fams) {
+        a.J = family_J(f, prob);
+        for (int s = 0; s < shape_count(f); ++s) {
+            HIP_TRY(launch_step(f, s, a, ctx->stream));  // warm-up
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int r = 0; r < reps; ++r) HIP_TRY(launch_step(f, s, a, ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_f = f;
+                best_s = s;
+            }
+        }
+    }
+    ctx->tune[key] = {best_f, best_s};
+    *family = best_f;
+    *shape = best_s;
+    return SB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Solve.
+
+int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const float *Y, int R, sb_result *res,
+                 hipStream_t s)
+{
+    const int N = prob->N, ld = prob->ld;
+    HIP_TRY(launch_energy(prob->J32, ld, X, ld, N, R, ctx->part, ctx->H, s));
+    std::vector<double> H(R);
+    HIP_TRY(hipMemcpyAsync(H.data(), ctx->H, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+    if (res->spins) {
+        HIP_TRY(launch_spins(X, ld, N, R, ctx->S, s));
+        HIP_TRY(hipMemcpyAsync(res->spins, ctx->S, (size_t)R * N, hipMemcpyDeviceToHost, s));
+    }
+    if (res->x_final) {
+        HIP_TRY(launch_compact(X, ld, N, R, ctx->tmp, s));
+        HIP_TRY(hipMemcpyAsync(res->x_final, ctx->tmp, sizeof(float) * (size_t)R * N, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (res->y_final) {
+        HIP_TRY(launch_compact(Y, ld, N, R, ctx->tmp, s));
+        HIP_TRY(hipMemcpyAsync(res->y_final, ctx->tmp, sizeof(float) * (size_t)R * N, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    int best = 0;
+    for (int r = 0; r < R; ++r) {
+        if (res->energy_f64) res->energy_f64[r] = H[r];
+        if (res->energy_i64) res->energy_i64[r] = prob->integer ? (int64_t)std::llround(H[r]) : 0;
+        if (H[r] < H[best]) best = r;
+    }
+    res->best_replica = best;
+    res->energy_exact = prob->integer ? 1 : 0;
+    return SB_OK;
+}
+
+int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)
+{
+    const int N = prob->N, ld = prob->ld, R = p->replicas;
+    hipStream_t s = ctx->stream;
+    int family = -1, shape = -1;
+    int rc;
+    if (p->precision != SB_PREC_ORDERED) {
+        rc = choose_kernel(ctx, prob, p->precision, p->variant, R, &family, &shape);
+        if (rc) return rc;
+    }
+    rc = ensure_workspace(ctx, (size_t)R * ld, R, N);
+    if (rc) return rc;
+    float *X = ctx->X0, *Xn = ctx->X1, *Y = ctx->Y;
+    HIP_TRY(launch_init(p->init, p->seed, N, R, ld, X, Y, Xn, s));
+    sb_step_coeffs c{};
+    if (p->variant == SB_ASB) {
+        // Local sub-steps of step 0 (m2_opt.cu:168, s = 0).
+        c.a0 = p->a0;
+        c.K = p->K;
+        c.dts = p->dt / (double)p->M;
+        c.M = p->M;
+        c.p_next = 0.0f;
+        HIP_TRY(launch_asb_local(c, N, R, ld, X, Y, s));
+    }
+    StepArgs a{};
+    a.J = p->precision == SB_PREC_ORDERED ? (const void *)prob->J32 : family_J(family, prob);
+    a.ldj = ld;
+    a.ldx = ld;
+    a.N = N;
+    a.R = R;
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    for (int k = 0; k < p->steps; ++k) {
+        sb_step_coeffs_for(p, c0, k, &a.c);
+        a.X = X;
+        a.Xn = Xn;
+        a.Y = Y;
+        hipError_t e = p->precision == SB_PREC_ORDERED ? launch_step_ordered(a, s)
+                                                       : launch_step(family, shape, a, s);
+        if (e != hipSuccess) return hip_fail(e, "launch_step");
+        std::swap(X, Xn);
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+    rc = read_results(ctx, prob, X, Y, R, res, s);
+    if (rc) return rc;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res->step_ms = ms;
+    res->c0_used = c0;
+    return SB_OK;
+}
+
+}  // namespace sbh
+
+extern "C" SB_API int sb_solve(sb_ctx *ctx, sb_problem *prob, const sb_params *params, sb_result *res)
+{
+    if (!ctx || !prob || !params || !res || prob->ctx != ctx) return SB_E_INVALID;
+    auto t0 = std::chrono::steady_clock::now();
+    double c0 = 0;
+    int rc = sbh::check_params(params, prob->N, prob->sigma, &c0);
+    if (rc) return rc;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    rc = sbh::solve_on_workspace(ctx, prob, params, c0, res);
+    if (rc) return rc;
+    res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return SB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Low-level operators on caller buffers.
+
+extern "C" SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, int32_t jfmt, int32_t N,
+                              int32_t ldj, const float *dX, float *dXn, float *dY, int32_t R, int32_t ldx,
+                              const sb_step_coeffs *c, void *stream)
+{
+    if (!ctx || !dJ || !dX || !dXn || !dY || !c || N < 1 || R < 1) return SB_E_INVALID;
+    if (ldj < N || ldx < N || ldj % kSpinPad || ldx % kSpinPad) return SB_E_INVALID;
+    if (c->variant < SB_ASB || c->variant > SB_DSB) return SB_E_INVALID;
+    int family;
+    if (precision == SB_PREC_FP32 || precision == SB_PREC_ORDERED) {
+        if (jfmt != SB_J_FP32) return SB_E_INVALID;
+        family = FAM_FP32;
+    } else if (precision == SB_PREC_SPLIT) {
+        if (jfmt == SB_J_BF16) family = FAM_SPLIT_BF16;
+        else if (jfmt == SB_J_FP8) family = c->variant == SB_DSB ? FAM_DSB_FP8 : FAM_SPLIT_FP8;
+        else return SB_E_INVALID;
+    } else {
+        return SB_E_INVALID;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    StepArgs a{};
+    a.J = dJ;
+    a.X = dX;
+    a.Xn = dXn;
+    a.Y = dY;
+    a.ldj = ldj;
+    a.ldx = ldx;
+    a.N = N;
+    a.R = R;
+    a.c = *c;
+    hipError_t e;
+    if (precision == SB_PREC_ORDERED) {
+        e = launch_step_ordered(a, s);
+    } else {
+        int shape = sbh::heuristic_shape(family, N, R);
+        e = launch_step(family, shape, a, s);
+    }
+    if (e != hipSuccess) return sbh::hip_fail(e, "sb_step");
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_init_state(sb_ctx *ctx, int32_t init, uint64_t seed, int32_t N, int32_t R, int32_t ldx,
+                                    float *dX, float *dY, void *stream)
+{
+    if (!ctx || !dX || !dY || N < 1 || R < 1 || ldx < N) return SB_E_INVALID;
+    if (init != SB_INIT_UNIFORM && init != SB_INIT_REFERENCE) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(launch_init(init, seed, N, R, ldx, dX, dY, nullptr, s));
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_asb_prologue(sb_ctx *ctx, const sb_params *p, int32_t N, int32_t R, int32_t ldx,
+                                      float *dX, float *dY, void *stream)
+{
+    if (!ctx || !p || !dX || !dY || N < 1 || R < 1 || ldx < N || p->M < 1) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    sb_step_coeffs c{};
+    c.a0 = p->a0;
+    c.K = p->K;
+    c.dts = p->dt / (double)p->M;
+    c.M = p->M;
+    c.p_next = 0.0f;
+    HIP_TRY(launch_asb_local(c, N, R, ldx, dX, dY, s));
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_energy(sb_ctx *ctx, const float *dJ, int32_t N, int32_t ldj, const float *dX, int32_t R,
+                                int32_t ldx, double *dH, void *stream)
+{
+    if (!ctx || !dJ || !dX || !dH || N < 1 || R < 1 || ldj < N || ldx < N) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    size_t need = energy_partial_elems(N, R);
+    int rc = sbh::grow(&ctx->part, &ctx->part_cap, need);
+    if (rc) return rc;
+    HIP_TRY(launch_energy(dJ, ldj, dX, ldx, N, R, ctx->part, dH, s));
+    return SB_OK;
+}

@@ -1,0 +1,73 @@
+// sb_host.h -- host-side state shared by the library's translation units.
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "sb_internal.h"
+
+struct sb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::recursive_mutex mu;
+    // Solve workspace (grown on demand, reused across calls).
+    float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;
+    size_t state_cap = 0;  // floats per buffer
+    double *H = nullptr, *part = nullptr;
+    size_t H_cap = 0, part_cap = 0;
+    int8_t *S = nullptr;
+    size_t S_cap = 0;
+    float *tmp = nullptr;
+    size_t tmp_cap = 0;
+    // Autotuned (family, shape) per (family set, N bucket, R bucket).
+    std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tune;
+    bool autotune = true;
+};
+
+struct sb_problem {
+    sb_ctx *ctx = nullptr;
+    int N = 0, ld = 0;
+    bool integer = false;
+    double sigma = 0.0, max_abs = 0.0;
+    float *J32 = nullptr;
+    uint16_t *Jbf16 = nullptr;
+    uint8_t *Jfp8 = nullptr;  // only when integer and |J| <= 16
+};
+
+namespace sbh {
+
+// Thread-local detail of the last failure (sb_last_error_message).
+void set_error(const std::string &msg);
+
+int hip_fail(hipError_t e, const char *what);
+
+// Validate params against a problem; fills the derived c0.
+int check_params(const sb_params *p, int N, double sigma, double *c0_out);
+
+// Pick the kernel family for a precision/variant/problem (before tuning).
+// Returns the list of families whose results are identical for this case;
+// the autotuner may choose any of them.
+std::vector<int> candidate_families(int precision, int variant, const sb_problem *prob);
+
+const void *family_J(int family, const sb_problem *prob);
+
+// Autotune (family, shape) for one problem size; cached per context.
+int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int variant, int R,
+                  int *family, int *shape);
+
+int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N);
+
+// Runs the full solve on the context's workspace; shared by sb_solve and the
+// server's single-request fallback.
+int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0,
+                       sb_result *res);
+
+// Reads energies/spins of R replicas from device state X ([R][ld]).
+int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const float *Y, int R,
+                 sb_result *res, hipStream_t s);
+
+}  // namespace sbh

@@ -1,0 +1,103 @@
+// sb_internal.h -- types shared by the gfx950 kernels (sb_kernels.hip) and the
+// host library (sb_api.cpp, sb_batch.cpp, sb_server.cpp).  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/sb.h"
+
+namespace sbk {
+
+// Spin padding of every device row (J rows and X/Y replica rows): all K loops
+// read up to round_up(N, kSpinPad) and rely on zeros there.
+constexpr int kSpinPad = 64;
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Arguments of one fused step over one instance.
+struct StepArgs {
+    const void *J;   // [N][ldj] in the kernel's J format
+    const float *X;  // [R][ldx] state read this step
+    float *Xn;       // [R][ldx] state written this step
+    float *Y;        // [R][ldx] updated in place
+    int ldj, ldx;
+    int N, R;
+    int tiles_m, tiles_n;  // row tiles x replica tiles of this launch
+    sb_step_coeffs c;
+};
+
+// Tile shapes.  A workgroup has WM x WN waves; each wave owns BM x BN blocks
+// of 32 x 32 (spins x replicas).  Every shape accumulates each F_i in the same
+// k order, so the choice never changes results.
+struct TileShape {
+    int wm, wn, bm, bn;
+    int mt() const { return 32 * bm * wm; }
+    int nt() const { return 32 * bn * wn; }
+    int threads() const { return 64 * wm * wn; }
+};
+
+// Kernel families.
+enum Family {
+    FAM_FP32 = 0,       // FP32 J x FP32 X on v_mfma_f32_32x32x2_f32
+    FAM_SPLIT_BF16 = 1, // BF16 J x BF16 hi/lo X (or BF16 sign X for dSB)
+    FAM_SPLIT_FP8 = 2,  // FP8 J widened to BF16 x BF16 hi/lo X
+    FAM_DSB_FP8 = 3,    // FP8 J x FP8 sign(X) on the f8f6f4 MFMA (dSB only)
+    FAM_COUNT = 4
+};
+
+// Number of compiled tile shapes per family and their table.
+int shape_count(int family);
+TileShape shape_at(int family, int idx);
+
+// Launchers (sb_kernels.hip).  Return a hipError_t.
+hipError_t launch_step(int family, int shape_idx, const StepArgs &a, hipStream_t s);
+hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s);
+hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X, float *Y,
+                       float *X2, hipStream_t s);
+hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, float *X, float *Y,
+                            hipStream_t s);
+hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N, int R,
+                         double *partial, double *H, hipStream_t s);
+size_t energy_partial_elems(int N, int R);
+hipError_t launch_spins(const float *X, int ldx, int N, int R, int8_t *S, hipStream_t s);
+hipError_t launch_compact(const float *X, int ldx, int N, int R, float *out, hipStream_t s);
+hipError_t launch_pack_bf16(const float *J, size_t n_elems, uint16_t *out, hipStream_t s);
+hipError_t launch_pack_fp8(const float *J, size_t n_elems, uint8_t *out, hipStream_t s);
+
+// ---- batched launch ------------------------------------------------------
+// One work item = one (instance, row tile, replica tile) of one step.
+struct BatchInstance {
+    const void *J;
+    const float *X0;   // ping-pong state buffers
+    float *X1;
+    float *Y;
+    int ldj, ldx;
+    int N, R;
+    int family, shape;  // kernel family and tile shape for this instance
+    int tiles_m, tiles_n;
+    int steps;          // schedule length S
+    int step_base;      // step index of the first launch of the current chunk
+    int variant, M;
+    float alpha_unused;
+    double dt, a0, c0, K;
+};
+
+struct BatchItem {
+    int inst;   // index into the instance table
+    int tile;   // tile index within the instance (tm + tiles_m * tn)
+};
+
+struct BatchArgs {
+    const BatchInstance *inst;  // device table
+    const BatchItem *items;     // device list, grouped per workgroup
+    const int *wg_begin;        // [workgroups + 1] item range of each workgroup
+    int chunk_step;             // step offset inside the chunk (0..)
+    int workgroups;
+};
+
+// Persistent batched step: every workgroup walks its own item list.
+hipError_t launch_batch_step(int family, const BatchArgs &a, hipStream_t s);
+
+}  // namespace sbk

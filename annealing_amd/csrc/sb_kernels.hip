@@ -1,0 +1,952 @@
+// sb_kernels.hip -- gfx950 kernels of the simulated-bifurcation solver.
+//
+// One fused launch per SB step computes the field F = J.X for a tile of
+// (spins x replicas) on MFMA, with J and X staged through LDS, and applies the
+// update of that tile in the same kernel:
+//   bSB/dSB (Goto et al. 2021): y += dt(-(a0-a)x + c0 F); x += dt a0 y; wall
+//   aSB (reference SB/CUDA_GPU/m2_opt.cu:58-75,168-169): kick y += dt c0 F,
+//        then the next step's M local sub-steps (FP64-evaluated, FP32-stored)
+// Data layout (HBM): J [N][ldj] row-major (FP32 / BF16 / FP8-E4M3), state
+// X, Y [R][ldx] FP32 replica-major with spins contiguous; ldj, ldx multiples
+// of 64 with zero padding.  X is double-buffered across steps, Y in place.
+//
+// Build flags: -O3 --offload-arch=gfx950 -ffp-contract=off (only the explicit
+// fmaf / fma calls fuse, matching the CPU restatement op for op).
+#include "sb_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace sbk {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Workgroup -> tile mapping.  Workgroup b is dispatched to XCD b % 8; give
+// each XCD a contiguous range of logical tiles so tiles that share replica
+// columns (tm fastest) run on one XCD and share its L2.
+__device__ __forceinline__ int round_up_dev(int v, int m) { return (v + m - 1) / m * m; }
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblk)
+{
+    int xcd = bid & 7, local = bid >> 3;
+    int per = nblk >> 3, rem = nblk & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+// ---------------------------------------------------------------------------
+// Update rules (single definition used by every kernel family).
+
+__device__ __forceinline__ void asb_local(float &x, float &y, float p, double a0, double K,
+                                          double dts, int M)
+{
+    // m2_opt.cu:66-71 with double constants: each statement in FP64, stored FP32.
+    for (int m = 0; m < M; ++m) {
+        double t = a0 * (double)y;
+        t = t * dts;
+        x = (float)((double)x + t);
+        double xd = (double)x;
+        double cube = K * xd;
+        cube = cube * xd;
+        cube = cube * xd;
+        double lin = a0 - (double)p;
+        lin = lin * xd;
+        double u = cube + lin;
+        u = dts * u;
+        y = (float)((double)y - u);
+    }
+}
+
+__device__ __forceinline__ void sb_update(const sb_step_coeffs &c, float F, float &x, float &y)
+{
+    if (c.variant == SB_ASB) {
+        y = __builtin_fmaf(c.beta, F, y);  // cublasSgemv(alpha = dt xi, beta = 1)
+        if (c.do_local) asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
+    } else {
+        float yn = __builtin_fmaf(c.beta, F, __builtin_fmaf(-c.alpha, x, y));
+        float xn = __builtin_fmaf(c.gamma, yn, x);
+        if (__builtin_fabsf(xn) > 1.0f) {
+            xn = __builtin_copysignf(1.0f, xn);
+            yn = 0.0f;
+        }
+        x = xn;
+        y = yn;
+    }
+}
+
+__device__ __forceinline__ float sgn1(float v) { return v > 0.0f ? 1.0f : -1.0f; }
+
+// Epilogue of one 32x32 accumulator block.  MFMA 32x32 output layout: lane l,
+// register v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32, so each lane
+// owns four runs of four consecutive spins of one replica: one float4 each.
+__device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &acc, int row_base,
+                                               int col_base, int lane)
+{
+    int r = col_base + (lane & 31);
+    if (r >= a.R) return;
+    const float *xr = a.X + (size_t)r * a.ldx;
+    float *xnr = a.Xn + (size_t)r * a.ldx;
+    float *yr = a.Y + (size_t)r * a.ldx;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        int i = row_base + 8 * g + 4 * (lane >> 5);
+        if (i >= a.N) continue;  // i < N implies i + 3 < ldx
+        v4f xv = *(const v4f *)(xr + i);
+        v4f yv = *(const v4f *)(yr + i);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float xe = xv[e], ye = yv[e];
+            sb_update(a.c, acc[4 * g + e], xe, ye);
+            xv[e] = xe;
+            yv[e] = ye;
+        }
+        *(v4f *)(xnr + i) = xv;
+        *(v4f *)(yr + i) = yv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 family: v_mfma_f32_32x32x2_f32.  K chunk of 32 staged through LDS with
+// double buffering.  Each lane reads a float4 of 4 consecutive k for its row
+// (A) and its replica (B); the four components feed four MFMAs, lane half
+// h = l/32 covering k = 8kk + 4h + c.  A and B use the same k assignment, so
+// the contraction is exact regardless of the hardware's k numbering.
+
+template <int WM, int WN, int BM, int BN>
+struct Fp32Tile {
+    static constexpr int MT = 32 * BM * WM, NT = 32 * BN * WN, NTHR = 64 * WM * WN;
+    static constexpr int KC = 32, LDS_LD = KC + 4;
+    static constexpr int LA = (MT * KC / 4 + NTHR - 1) / NTHR;
+    static constexpr int LB = (NT * KC / 4 + NTHR - 1) / NTHR;
+};
+
+template <int WM, int WN, int BM, int BN>
+__device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *smem)
+{
+    using T = Fp32Tile<WM, WN, BM, BN>;
+    constexpr int MT = T::MT, NT = T::NT, NTHR = T::NTHR, KC = T::KC, LD = T::LDS_LD;
+    float *As = smem;                  // [2][MT][LD]
+    float *Bs = smem + 2 * MT * LD;    // [2][NT][LD]
+    const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
+    const int row0 = tm * MT, col0 = tn * NT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const float *J = (const float *)a.J;
+    const bool sgn = a.c.variant == SB_DSB;
+    const int Kp = round_up_dev(a.N, KC);
+
+    v4f ra[T::LA], rb[T::LB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < T::LA; ++u) {
+            int q = tid + u * NTHR;
+            int rr = q / (KC / 4), c4 = q % (KC / 4);
+            int gi = row0 + rr;
+            v4f v = {0.f, 0.f, 0.f, 0.f};
+            if (q < MT * KC / 4 && gi < a.N) v = *(const v4f *)(J + (size_t)gi * a.ldj + k0 + 4 * c4);
+            ra[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < T::LB; ++u) {
+            int q = tid + u * NTHR;
+            int rr = q / (KC / 4), c4 = q % (KC / 4);
+            int gr = col0 + rr;
+            v4f v = {0.f, 0.f, 0.f, 0.f};
+            if (q < NT * KC / 4 && gr < a.R) {
+                v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c4);
+                if (sgn) v = (v4f){sgn1(v[0]), sgn1(v[1]), sgn1(v[2]), sgn1(v[3])};
+            }
+            rb[u] = v;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < T::LA; ++u) {
+            int q = tid + u * NTHR;
+            if (q < MT * KC / 4) {
+                int rr = q / (KC / 4), c4 = q % (KC / 4);
+                *(v4f *)(As + (buf * MT + rr) * LD + 4 * c4) = ra[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < T::LB; ++u) {
+            int q = tid + u * NTHR;
+            if (q < NT * KC / 4) {
+                int rr = q / (KC / 4), c4 = q % (KC / 4);
+                *(v4f *)(Bs + (buf * NT + rr) * LD + 4 * c4) = rb[u];
+            }
+        }
+    };
+
+    v16f acc[BM][BN];
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[bm][bn][v] = 0.0f;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int arow = wm * 32 * BM + (lane & 31), bcol = wn * 32 * BN + (lane & 31);
+    const int koff = 4 * (lane >> 5);
+    int buf = 0;
+    for (int k0 = 0; k0 < Kp; k0 += KC) {
+        const bool more = k0 + KC < Kp;
+        if (more) gload(k0 + KC);
+        const float *Ab = As + buf * MT * LD;
+        const float *Bb = Bs + buf * NT * LD;
+#pragma unroll
+        for (int kk = 0; kk < KC / 8; ++kk) {
+            v4f fa[BM], fb[BN];
+#pragma unroll
+            for (int bm = 0; bm < BM; ++bm) fa[bm] = *(const v4f *)(Ab + (arow + 32 * bm) * LD + 8 * kk + koff);
+#pragma unroll
+            for (int bn = 0; bn < BN; ++bn) fb[bn] = *(const v4f *)(Bb + (bcol + 32 * bn) * LD + 8 * kk + koff);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+                    for (int bn = 0; bn < BN; ++bn)
+                        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[bm][c], fb[bn][c], acc[bm][bn], 0, 0, 0);
+        }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+            epilogue_block(a, acc[bm][bn], row0 + wm * 32 * BM + 32 * bm, col0 + wn * 32 * BN + 32 * bn, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Split family: v_mfma_f32_32x32x16_bf16.  K chunk of 64.  A = J in BF16 (read
+// as BF16, or as FP8 E4M3 widened to BF16 while staging).  B = X split into
+// BF16 hi = bf16(x) and lo = bf16(x - hi) while staging, two MFMA passes into
+// one FP32 accumulator per k16 group (hi then lo); dSB stages sign(x) once.
+// Lane half h reads 8 consecutive k (16 bytes) at 16kk + 8h for A and B.
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
+}
+__device__ __forceinline__ float bf2f(unsigned short h)
+{
+    return __builtin_bit_cast(float, (unsigned int)h << 16);
+}
+
+// FP8 E4M3 byte -> BF16 bits for normal numbers and zero (the packer stores
+// only integers |J| <= 16, which are all normal): exponent bias 7 -> 127 is an
+// add of 120 to the exponent field, mantissa 3 -> 7 bits a shift.
+__device__ __forceinline__ unsigned int fp8pair_to_bf16x2(unsigned int b2)
+{
+    unsigned int lo = b2 & 0xffu, hi = (b2 >> 8) & 0xffu;
+    unsigned int ml = lo & 0x7fu, mh = hi & 0x7fu;
+    unsigned int rl = ((lo & 0x80u) << 8) | (ml ? (ml << 4) + (120u << 7) : 0u);
+    unsigned int rh = ((hi & 0x80u) << 8) | (mh ? (mh << 4) + (120u << 7) : 0u);
+    return rl | (rh << 16);
+}
+
+template <int WM, int WN, int BM, int BN>
+struct SplitTile {
+    static constexpr int MT = 32 * BM * WM, NT = 32 * BN * WN, NTHR = 64 * WM * WN;
+    static constexpr int KC = 64, LDS_LD = KC + 8;  // in bf16 elements
+};
+
+template <int WM, int WN, int BM, int BN, bool FP8J>
+__device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned short *smem)
+{
+    using T = SplitTile<WM, WN, BM, BN>;
+    constexpr int MT = T::MT, NT = T::NT, NTHR = T::NTHR, KC = T::KC, LD = T::LDS_LD;
+    // A: 16-byte global loads of 8 BF16 or 16 FP8.
+    constexpr int A_PER = FP8J ? 16 : 8;               // elements per global load
+    constexpr int A_LOADS = MT * KC / A_PER;
+    constexpr int LA = (A_LOADS + NTHR - 1) / NTHR;
+    constexpr int B_LOADS = NT * KC / 4;                // float4 loads of X
+    constexpr int LB = (B_LOADS + NTHR - 1) / NTHR;
+    unsigned short *As = smem;                          // [2][MT][LD]
+    unsigned short *Bh = smem + 2 * MT * LD;            // [2][NT][LD]
+    unsigned short *Bl = Bh + 2 * NT * LD;              // [2][NT][LD]
+    const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
+    const int row0 = tm * MT, col0 = tn * NT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const bool sgn = a.c.variant == SB_DSB;
+    const int Kp = round_up_dev(a.N, KC);
+
+    u32x4 ra[LA];
+    v4f rb[LB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            int q = tid + u * NTHR;
+            int rr = q / (KC / A_PER), c = q % (KC / A_PER);
+            int gi = row0 + rr;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (q < A_LOADS && gi < a.N) {
+                if (FP8J)
+                    v = *(const u32x4 *)((const unsigned char *)a.J + (size_t)gi * a.ldj + k0 + 16 * c);
+                else
+                    v = *(const u32x4 *)((const unsigned short *)a.J + (size_t)gi * a.ldj + k0 + 8 * c);
+            }
+            ra[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            int q = tid + u * NTHR;
+            int rr = q / (KC / 4), c4 = q % (KC / 4);
+            int gr = col0 + rr;
+            v4f v = {0.f, 0.f, 0.f, 0.f};
+            if (q < B_LOADS && gr < a.R) v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c4);
+            rb[u] = v;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            int q = tid + u * NTHR;
+            if (q < A_LOADS) {
+                int rr = q / (KC / A_PER), c = q % (KC / A_PER);
+                unsigned short *dst = As + (buf * MT + rr) * LD + A_PER * c;
+                if (FP8J) {
+                    u32x4 lo4, hi4;
+                    lo4[0] = fp8pair_to_bf16x2(ra[u][0]);
+                    lo4[1] = fp8pair_to_bf16x2(ra[u][0] >> 16);
+                    lo4[2] = fp8pair_to_bf16x2(ra[u][1]);
+                    lo4[3] = fp8pair_to_bf16x2(ra[u][1] >> 16);
+                    hi4[0] = fp8pair_to_bf16x2(ra[u][2]);
+                    hi4[1] = fp8pair_to_bf16x2(ra[u][2] >> 16);
+                    hi4[2] = fp8pair_to_bf16x2(ra[u][3]);
+                    hi4[3] = fp8pair_to_bf16x2(ra[u][3] >> 16);
+                    *(u32x4 *)dst = lo4;
+                    *(u32x4 *)(dst + 8) = hi4;
+                } else {
+                    *(u32x4 *)dst = ra[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            int q = tid + u * NTHR;
+            if (q < B_LOADS) {
+                int rr = q / (KC / 4), c4 = q % (KC / 4);
+                u16x4 h, l;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float v = rb[u][e];
+                    if (sgn) {
+                        h[e] = v > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
+                        l[e] = 0;
+                    } else {
+                        unsigned short hb = f2bf(v);
+                        h[e] = hb;
+                        l[e] = f2bf(v - bf2f(hb));
+                    }
+                }
+                *(u16x4 *)(Bh + (buf * NT + rr) * LD + 4 * c4) = h;
+                if (!sgn) *(u16x4 *)(Bl + (buf * NT + rr) * LD + 4 * c4) = l;
+            }
+        }
+    };
+
+    v16f acc[BM][BN];
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[bm][bn][v] = 0.0f;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int arow = wm * 32 * BM + (lane & 31), bcol = wn * 32 * BN + (lane & 31);
+    const int koff = 8 * (lane >> 5);
+    int buf = 0;
+    for (int k0 = 0; k0 < Kp; k0 += KC) {
+        const bool more = k0 + KC < Kp;
+        if (more) gload(k0 + KC);
+        const unsigned short *Ab = As + buf * MT * LD;
+        const unsigned short *Bhb = Bh + buf * NT * LD;
+        const unsigned short *Blb = Bl + buf * NT * LD;
+#pragma unroll
+        for (int kk = 0; kk < KC / 16; ++kk) {
+            bf16x8 fa[BM], fh[BN], fl[BN];
+#pragma unroll
+            for (int bm = 0; bm < BM; ++bm)
+                fa[bm] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ab + (arow + 32 * bm) * LD + 16 * kk + koff));
+#pragma unroll
+            for (int bn = 0; bn < BN; ++bn) {
+                fh[bn] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Bhb + (bcol + 32 * bn) * LD + 16 * kk + koff));
+                if (!sgn) fl[bn] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Blb + (bcol + 32 * bn) * LD + 16 * kk + koff));
+            }
+#pragma unroll
+            for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+                for (int bn = 0; bn < BN; ++bn)
+                    acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[bm], fh[bn], acc[bm][bn], 0, 0, 0);
+            if (!sgn) {
+#pragma unroll
+                for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+                    for (int bn = 0; bn < BN; ++bn)
+                        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[bm], fl[bn], acc[bm][bn], 0, 0, 0);
+            }
+        }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+            epilogue_block(a, acc[bm][bn], row0 + wm * 32 * BM + 32 * bm, col0 + wn * 32 * BN + 32 * bn, lane);
+}
+
+// ---------------------------------------------------------------------------
+// dSB FP8 family: FP8 J x FP8 sign(x) on v_mfma_scale_f32_32x32x64_f8f6f4
+// with unit scales.  Products are +-J_ij exactly and the FP32 sums of
+// integers below 2^24 are exact, so F equals the FP32 family's F bit for bit.
+// K chunk of 128 bytes; lane half h reads 32 consecutive k at 64kk + 32h.
+
+template <int WM, int WN, int BM, int BN>
+struct Fp8Tile {
+    static constexpr int MT = 32 * BM * WM, NT = 32 * BN * WN, NTHR = 64 * WM * WN;
+    static constexpr int KC = 128, LDS_LD = KC + 16;  // bytes
+};
+
+template <int WM, int WN, int BM, int BN>
+__device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsigned char *smem)
+{
+    using T = Fp8Tile<WM, WN, BM, BN>;
+    constexpr int MT = T::MT, NT = T::NT, NTHR = T::NTHR, KC = T::KC, LD = T::LDS_LD;
+    constexpr int A_LOADS = MT * KC / 16;
+    constexpr int LA = (A_LOADS + NTHR - 1) / NTHR;
+    constexpr int B_LOADS = NT * KC / 4;  // float4 loads of X -> 4 sign bytes
+    constexpr int LB = (B_LOADS + NTHR - 1) / NTHR;
+    unsigned char *As = smem;               // [2][MT][LD]
+    unsigned char *Bs = smem + 2 * MT * LD; // [2][NT][LD]
+    const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
+    const int row0 = tm * MT, col0 = tn * NT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    // K loop bound: multiple of KC inside the ld padding (ld is a multiple of 64;
+    // reads past ld are avoided by clamping the last chunk's length).
+    const int Kp = round_up_dev(a.N, 64);
+
+    u32x4 ra[LA];
+    unsigned int rb[LB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            int q = tid + u * NTHR;
+            int rr = q / (KC / 16), c = q % (KC / 16);
+            int gi = row0 + rr, gk = k0 + 16 * c;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (q < A_LOADS && gi < a.N && gk < Kp)
+                v = *(const u32x4 *)((const unsigned char *)a.J + (size_t)gi * a.ldj + gk);
+            ra[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            int q = tid + u * NTHR;
+            int rr = q / (KC / 4), c4 = q % (KC / 4);
+            int gr = col0 + rr, gk = k0 + 4 * c4;
+            unsigned int s4 = 0u;  // FP8 zero bytes beyond N: J is zero there anyway
+            if (q < B_LOADS && gr < a.R && gk < Kp) {
+                v4f v = *(const v4f *)(a.X + (size_t)gr * a.ldx + gk);
+                // E4M3 +1.0 = 0x38, -1.0 = 0xB8
+#pragma unroll
+                for (int e = 0; e < 4; ++e) s4 |= (v[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
+            }
+            rb[u] = s4;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < LA; ++u) {
+            int q = tid + u * NTHR;
+            if (q < A_LOADS) {
+                int rr = q / (KC / 16), c = q % (KC / 16);
+                *(u32x4 *)(As + (buf * MT + rr) * LD + 16 * c) = ra[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            int q = tid + u * NTHR;
+            if (q < B_LOADS) {
+                int rr = q / (KC / 4), c4 = q % (KC / 4);
+                *(unsigned int *)(Bs + (buf * NT + rr) * LD + 4 * c4) = rb[u];
+            }
+        }
+    };
+
+    v16f acc[BM][BN];
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[bm][bn][v] = 0.0f;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int arow = wm * 32 * BM + (lane & 31), bcol = wn * 32 * BN + (lane & 31);
+    const int koff = 32 * (lane >> 5);
+    int buf = 0;
+    for (int k0 = 0; k0 < Kp; k0 += KC) {
+        const bool more = k0 + KC < Kp;
+        if (more) gload(k0 + KC);
+        const unsigned char *Ab = As + buf * MT * LD;
+        const unsigned char *Bb = Bs + buf * NT * LD;
+#pragma unroll
+        for (int kk = 0; kk < KC / 64; ++kk) {
+            i32x8 fa[BM], fb[BN];
+#pragma unroll
+            for (int bm = 0; bm < BM; ++bm) {
+                const unsigned char *p = Ab + (arow + 32 * bm) * LD + 64 * kk + koff;
+                u32x4 x0 = *(const u32x4 *)p, x1 = *(const u32x4 *)(p + 16);
+                fa[bm] = (i32x8){(int)x0[0], (int)x0[1], (int)x0[2], (int)x0[3],
+                                 (int)x1[0], (int)x1[1], (int)x1[2], (int)x1[3]};
+            }
+#pragma unroll
+            for (int bn = 0; bn < BN; ++bn) {
+                const unsigned char *p = Bb + (bcol + 32 * bn) * LD + 64 * kk + koff;
+                u32x4 x0 = *(const u32x4 *)p, x1 = *(const u32x4 *)(p + 16);
+                fb[bn] = (i32x8){(int)x0[0], (int)x0[1], (int)x0[2], (int)x0[3],
+                                 (int)x1[0], (int)x1[1], (int)x1[2], (int)x1[3]};
+            }
+#pragma unroll
+            for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+                for (int bn = 0; bn < BN; ++bn)
+                    acc[bm][bn] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        fa[bm], fb[bn], acc[bm][bn], 0, 0, 0, 127, 0, 127);
+        }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+            epilogue_block(a, acc[bm][bn], row0 + wm * 32 * BM + 32 * bm, col0 + wn * 32 * BN + 32 * bn, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel entry points per family and shape.
+
+template <int WM, int WN, int BM, int BN>
+__global__ __launch_bounds__(64 * WM * WN) void k_step_fp32(StepArgs a)
+{
+    using T = Fp32Tile<WM, WN, BM, BN>;
+    __shared__ __attribute__((aligned(16))) float smem[2 * (T::MT + T::NT) * T::LDS_LD];
+    fp32_tile<WM, WN, BM, BN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+}
+
+template <int WM, int WN, int BM, int BN, bool FP8J>
+__global__ __launch_bounds__(64 * WM * WN) void k_step_split(StepArgs a)
+{
+    using T = SplitTile<WM, WN, BM, BN>;
+    __shared__ __attribute__((aligned(16))) unsigned short smem[2 * (T::MT + 2 * T::NT) * T::LDS_LD];
+    split_tile<WM, WN, BM, BN, FP8J>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+}
+
+template <int WM, int WN, int BM, int BN>
+__global__ __launch_bounds__(64 * WM * WN) void k_step_dsb_fp8(StepArgs a)
+{
+    using T = Fp8Tile<WM, WN, BM, BN>;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[2 * (T::MT + T::NT) * T::LDS_LD];
+    dsb_fp8_tile<WM, WN, BM, BN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+}
+
+// Ordered-sum debug kernel: one thread per (spin, replica), F_i summed over
+// j = 0..N-1 with fmaf in order (Bifurcation_m2.cu.c:93-97).
+__global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= a.N || r >= a.R) return;
+    const float *Jr = (const float *)a.J + (size_t)i * a.ldj;
+    const float *xr = a.X + (size_t)r * a.ldx;
+    const bool sgn = a.c.variant == SB_DSB;
+    float F = 0.0f;
+    for (int j = 0; j < a.N; ++j) {
+        float v = xr[j];
+        if (sgn) v = sgn1(v);
+        F = __builtin_fmaf(Jr[j], v, F);
+    }
+    float x = xr[i], y = a.Y[(size_t)r * a.ldx + i];
+    sb_update(a.c, F, x, y);
+    a.Xn[(size_t)r * a.ldx + i] = x;
+    a.Y[(size_t)r * a.ldx + i] = y;
+}
+
+// ---------------------------------------------------------------------------
+// Shape tables.  All shapes use 4 waves (256 threads).
+
+#define SB_FP32_SHAPES(X) X(2, 2, 1, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1)
+#define SB_SPLIT_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 1, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1)
+#define SB_FP8_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 1, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1)
+
+#define SB_SHAPE_ENTRY(wm, wn, bm, bn) TileShape{wm, wn, bm, bn},
+static const TileShape kFp32Shapes[] = {SB_FP32_SHAPES(SB_SHAPE_ENTRY)};
+static const TileShape kSplitShapes[] = {SB_SPLIT_SHAPES(SB_SHAPE_ENTRY)};
+static const TileShape kFp8Shapes[] = {SB_FP8_SHAPES(SB_SHAPE_ENTRY)};
+#undef SB_SHAPE_ENTRY
+
+int shape_count(int family)
+{
+    switch (family) {
+    case FAM_FP32: return (int)(sizeof(kFp32Shapes) / sizeof(kFp32Shapes[0]));
+    case FAM_SPLIT_BF16:
+    case FAM_SPLIT_FP8: return (int)(sizeof(kSplitShapes) / sizeof(kSplitShapes[0]));
+    case FAM_DSB_FP8: return (int)(sizeof(kFp8Shapes) / sizeof(kFp8Shapes[0]));
+    default: return 0;
+    }
+}
+
+TileShape shape_at(int family, int idx)
+{
+    switch (family) {
+    case FAM_FP32: return kFp32Shapes[idx];
+    case FAM_SPLIT_BF16:
+    case FAM_SPLIT_FP8: return kSplitShapes[idx];
+    default: return kFp8Shapes[idx];
+    }
+}
+
+typedef void (*StepKernel)(StepArgs);
+
+#define SB_FP32_PTR(wm, wn, bm, bn) &k_step_fp32<wm, wn, bm, bn>,
+#define SB_SPLIT_BF16_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false>,
+#define SB_SPLIT_FP8_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true>,
+#define SB_DSB_FP8_PTR(wm, wn, bm, bn) &k_step_dsb_fp8<wm, wn, bm, bn>,
+static const StepKernel kFp32Kernels[] = {SB_FP32_SHAPES(SB_FP32_PTR)};
+static const StepKernel kSplitBf16Kernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_BF16_PTR)};
+static const StepKernel kSplitFp8Kernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_FP8_PTR)};
+static const StepKernel kDsbFp8Kernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PTR)};
+
+hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStream_t s)
+{
+    if (shape_idx < 0 || shape_idx >= shape_count(family)) return hipErrorInvalidValue;
+    TileShape sh = shape_at(family, shape_idx);
+    StepArgs a = a_in;
+    a.tiles_m = (a.N + sh.mt() - 1) / sh.mt();
+    a.tiles_n = (a.R + sh.nt() - 1) / sh.nt();
+    int grid = a.tiles_m * a.tiles_n;
+    if (grid <= 0) return hipSuccess;
+    StepKernel k = nullptr;
+    switch (family) {
+    case FAM_FP32: k = kFp32Kernels[shape_idx]; break;
+    case FAM_SPLIT_BF16: k = kSplitBf16Kernels[shape_idx]; break;
+    case FAM_SPLIT_FP8: k = kSplitFp8Kernels[shape_idx]; break;
+    case FAM_DSB_FP8: k = kDsbFp8Kernels[shape_idx]; break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(sh.threads()), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
+{
+    if (a.N <= 0 || a.R <= 0) return hipSuccess;
+    dim3 grid((a.N + 255) / 256, a.R);
+    hipLaunchKernelGGL(k_step_ordered, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Initial state.
+
+__device__ __forceinline__ uint32_t xorshift32(uint32_t &st)
+{
+    uint32_t v = st;
+    v ^= v << 13;
+    v ^= v >> 17;
+    v ^= v << 5;
+    st = v;
+    return v;
+}
+
+// prepare_points value (m2_opt.cu:54-55): (float)v / 2^32 exactly, / 10.0 in FP64.
+__device__ __forceinline__ float ref_draw_value(uint32_t v)
+{
+    float q = (float)v * 0x1p-32f;
+    return (float)((double)q / 10.0);
+}
+
+__device__ __forceinline__ uint32_t keyed_u24(uint64_t seed, uint32_t replica, uint32_t spin,
+                                              uint32_t comp)
+{
+    uint64_t key = ((uint64_t)replica << 32) | (uint64_t)spin;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (2ull * key + (uint64_t)comp + 1ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (uint32_t)(z >> 40);
+}
+
+__device__ __forceinline__ float keyed_value(uint32_t u24)
+{
+    float v = (float)u24 * 0x1p-23f;
+    v = v - 1.0f;
+    return v * 0.1f;
+}
+
+__global__ __launch_bounds__(256) void k_init_uniform(uint64_t seed, int N, int R, int ldx, float *X,
+                                                      float *Y, float *X2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= ldx || r >= R) return;
+    size_t o = (size_t)r * ldx + i;
+    float x = 0.0f, y = 0.0f;
+    if (i < N) {
+        x = keyed_value(keyed_u24(seed, (uint32_t)r, (uint32_t)i, 0u));
+        y = keyed_value(keyed_u24(seed, (uint32_t)r, (uint32_t)i, 1u));
+    }
+    X[o] = x;
+    Y[o] = y;
+    if (X2) X2[o] = 0.0f;
+}
+
+// One thread per spin walks its xorshift32 stream over the replicas, as the
+// reference's trial loop does (m2_opt.cu:136-137,152-154).
+__global__ __launch_bounds__(256) void k_init_reference(uint32_t seed, int N, int R, int ldx, float *X,
+                                                        float *Y, float *X2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ldx) return;
+    uint32_t st = (uint32_t)i + seed * (uint32_t)N;
+    for (int r = 0; r < R; ++r) {
+        size_t o = (size_t)r * ldx + i;
+        float x = 0.0f, y = 0.0f;
+        if (i < N) {
+            x = ref_draw_value(xorshift32(st));
+            y = ref_draw_value(xorshift32(st));
+        }
+        X[o] = x;
+        Y[o] = y;
+        if (X2) X2[o] = 0.0f;
+    }
+}
+
+hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X, float *Y, float *X2,
+                       hipStream_t s)
+{
+    if (R <= 0 || ldx <= 0) return hipSuccess;
+    if (kind == SB_INIT_REFERENCE) {
+        hipLaunchKernelGGL(k_init_reference, dim3((ldx + 255) / 256), dim3(256), 0, s, (uint32_t)seed, N, R,
+                           ldx, X, Y, X2);
+    } else {
+        hipLaunchKernelGGL(k_init_uniform, dim3((ldx + 255) / 256, R), dim3(256), 0, s, seed, N, R, ldx, X,
+                           Y, X2);
+    }
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_asb_local(sb_step_coeffs c, int N, int R, int ldx, float *X,
+                                                   float *Y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= N || r >= R) return;
+    size_t o = (size_t)r * ldx + i;
+    float x = X[o], y = Y[o];
+    asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
+    X[o] = x;
+    Y[o] = y;
+}
+
+hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, float *X, float *Y,
+                            hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_asb_local, dim3((N + 255) / 256, R), dim3(256), 0, s, c, N, R, ldx, X, Y);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Energy: H_r = -1/2 sum_i s_i G_i with G = J s, in FP64.  Tile 64 spins x 64
+// replicas per workgroup; every product is +-J_ij (exact), so for integer J
+// all partial sums are exact integers below 2^53 and the result is exact.
+// Deterministic: fixed per-thread order and a fixed reduction tree.
+
+constexpr int EN_T = 64, EN_KC = 32;
+
+__global__ __launch_bounds__(256) void k_energy_partial(const float *J, int ldj, const float *X, int ldx,
+                                                        int N, int R, double *part)
+{
+    __shared__ float Js[EN_KC][EN_T + 1];
+    __shared__ float Ss[EN_KC][EN_T + 1];
+    __shared__ double red[16][EN_T + 1];
+    const int tiles_m = (N + EN_T - 1) / EN_T;
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int row0 = tm * EN_T, col0 = tn * EN_T;
+    const int tid = threadIdx.x, ti = tid & 15, tr = tid >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int j0 = 0; j0 < N; j0 += EN_KC) {
+        for (int q = tid; q < EN_T * EN_KC; q += 256) {
+            int rr = q / EN_KC, c = q % EN_KC;
+            int gi = row0 + rr, gj = j0 + c;
+            Js[c][rr] = (gi < N && gj < N) ? J[(size_t)gi * ldj + gj] : 0.0f;
+            int gr = col0 + rr;
+            Ss[c][rr] = (gr < R && gj < N) ? sgn1(X[(size_t)gr * ldx + gj]) : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int c = 0; c < EN_KC; ++c) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = Js[c][ti * 4 + u];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bv[v] = Ss[c][tr * 4 + v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] += (double)av[u] * (double)bv[v];
+        }
+        __syncthreads();
+    }
+    // s_i * G_i summed over this thread's 4 rows, per replica.
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        int gr = col0 + tr * 4 + v;
+        double sum = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int gi = row0 + ti * 4 + u;
+            if (gi < N && gr < R) sum += sgn1(X[(size_t)gr * ldx + gi]) > 0.0f ? acc[u][v] : -acc[u][v];
+        }
+        red[ti][tr * 4 + v] = sum;
+    }
+    __syncthreads();
+    if (tid < EN_T) {
+        double s = 0.0;
+        for (int t = 0; t < 16; ++t) s += red[t][tid];
+        int gr = col0 + tid;
+        if (gr < R) part[(size_t)tm * R + gr] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(const double *part, int tiles_m, int R, double *H)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    double s = 0.0;
+    for (int t = 0; t < tiles_m; ++t) s += part[(size_t)t * R + r];
+    H[r] = -0.5 * s;
+}
+
+size_t energy_partial_elems(int N, int R)
+{
+    return (size_t)((N + EN_T - 1) / EN_T) * (size_t)R;
+}
+
+hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N, int R, double *partial,
+                         double *H, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    int tiles_m = (N + EN_T - 1) / EN_T, tiles_n = (R + EN_T - 1) / EN_T;
+    hipLaunchKernelGGL(k_energy_partial, dim3(tiles_m * tiles_n), dim3(256), 0, s, J, ldj, X, ldx, N, R,
+                       partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_energy_final, dim3((R + 255) / 256), dim3(256), 0, s, partial, tiles_m, R, H);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Readout and packing.
+
+__global__ __launch_bounds__(256) void k_spins(const float *X, int ldx, int N, int R, int8_t *S)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= N || r >= R) return;
+    S[(size_t)r * N + i] = X[(size_t)r * ldx + i] > 0.0f ? (int8_t)1 : (int8_t)-1;
+}
+
+hipError_t launch_spins(const float *X, int ldx, int N, int R, int8_t *S, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spins, dim3((N + 255) / 256, R), dim3(256), 0, s, X, ldx, N, R, S);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_compact(const float *X, int ldx, int N, int R, float *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (i >= N || r >= R) return;
+    out[(size_t)r * N + i] = X[(size_t)r * ldx + i];
+}
+
+hipError_t launch_compact(const float *X, int ldx, int N, int R, float *out, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact, dim3((N + 255) / 256, R), dim3(256), 0, s, X, ldx, N, R, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_pack_bf16(const float *J, size_t n, uint16_t *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = f2bf(J[i]);
+}
+
+// Integer |v| <= 16 -> E4M3 bits (v = m * 2^e with a 3-bit mantissa).
+__device__ __forceinline__ uint8_t int_to_e4m3(float v)
+{
+    if (v == 0.0f) return 0;
+    uint32_t b = __builtin_bit_cast(uint32_t, v);
+    uint32_t sign = (b >> 31) & 1u;
+    int e = (int)((b >> 23) & 0xffu) - 127;  // 0..4 for 1..16
+    uint32_t m = (b >> 20) & 0x7u;           // top 3 mantissa bits
+    return (uint8_t)((sign << 7) | ((uint32_t)(e + 7) << 3) | m);
+}
+
+__global__ __launch_bounds__(256) void k_pack_fp8(const float *J, size_t n, uint8_t *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = int_to_e4m3(J[i]);
+}
+
+static unsigned pack_grid(size_t n)
+{
+    size_t g = (n + 255) / 256;
+    return (unsigned)(g < 65536 ? g : 65536);
+}
+
+hipError_t launch_pack_bf16(const float *J, size_t n, uint16_t *out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_bf16, dim3(pack_grid(n)), dim3(256), 0, s, J, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_fp8(const float *J, size_t n, uint8_t *out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_fp8, dim3(pack_grid(n)), dim3(256), 0, s, J, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace sbk

@@ -1,0 +1,260 @@
+/*
+ * sb.h -- C ABI of the MI355X (gfx950) simulated-bifurcation solver.
+ *
+ * Drop-in for the reference's SB/CUDA_GPU solver (strongshih/Annealing).
+ * Paths below are relative to the reference repository root.
+ *
+ * The reference has two boundaries on this path:
+ *   - the process boundary `./opt <instance>` (SB/CUDA_GPU/m2_opt.cu:100-103),
+ *     which reads an edge list (:111-123) and writes one energy per trial to
+ *     output.txt (:206-210).  The CLI annealing_amd/bin/sb_solve keeps it and
+ *     is built on sb_problem_from_edgelist_file + sb_solve below.
+ *   - the in-process operator pair per step: UpdateTwice<<<N/64,64>>>(x, y, p)
+ *     (m2_opt.cu:168) followed by cublasSgemv(handle, OP_N, N, N, &coeff1,
+ *     J, N, x, 1, &coeff2, y, 1) (m2_opt.cu:169).  sb_step replaces the pair
+ *     with one fused launch over R replicas.
+ *
+ * Conventions: every entry point returns an sb_status (0 = success) and never
+ * exits the process; the library owns device buffers it allocates, the caller
+ * owns host buffers.  No torch or HIP types appear in signatures; streams are
+ * passed as void* (a hipStream_t, or NULL for the context's own stream).
+ * One sb_ctx may be used from several host threads (calls are serialised per
+ * context).
+ */
+#ifndef ANNEALING_AMD_SB_H
+#define ANNEALING_AMD_SB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define SB_API __attribute__((visibility("default")))
+#else
+#define SB_API
+#endif
+
+typedef enum sb_status {
+    SB_OK = 0,
+    SB_E_INVALID = 1,     /* bad argument, i == j edge, unsupported combination */
+    SB_E_IO = 2,          /* file cannot be opened / read                       */
+    SB_E_PARSE = 3,       /* malformed edge list                                */
+    SB_E_RANGE = 4,       /* spin index outside [0, N)                          */
+    SB_E_INEXACT = 5,     /* J cannot be stored exactly in the requested format */
+    SB_E_NOMEM = 6,       /* device or host allocation failed                   */
+    SB_E_HIP = 7,         /* a HIP runtime call failed                          */
+    SB_E_BUDGET = 8,      /* request exceeds the server's HBM budget            */
+    SB_E_NODEVICE = 9,    /* no usable GPU                                      */
+    SB_E_TIMEOUT = 10,    /* sb_server_wait timed out (request still running)   */
+    SB_E_CANCELLED = 11,  /* request was cancelled                              */
+    SB_E_NOTFOUND = 12,   /* unknown request ticket                             */
+    SB_E_SHUTDOWN = 13    /* server is shutting down                            */
+} sb_status;
+
+/* Update rule.  SB_ASB is the reference's adiabatic SB with Kerr term and M
+ * local sub-steps per coupling kick (m2_opt.cu:58-75,168-169).  SB_BSB and
+ * SB_DSB are ballistic and discrete SB (Goto et al., Sci. Adv. 7:eabe7953,
+ * 2021), which the reference does not implement. */
+typedef enum sb_variant { SB_ASB = 0, SB_BSB = 1, SB_DSB = 2 } sb_variant;
+
+/* Arithmetic of the field F = J.X.
+ *   SB_PREC_FP32    FP32 J and X on FP32 MFMA (the plain path).
+ *   SB_PREC_SPLIT   J in FP8 E4M3 when integer-valued with |J| <= 16, else
+ *                   BF16; X split into BF16 hi + lo; FP32 accumulation.
+ *                   dSB with FP8 J runs one exact FP8 x FP8 pass.
+ *   SB_PREC_ORDERED FP32, each F_i summed sequentially over j with fmaf, the
+ *                   order of the reference's naive kernel
+ *                   (Bifurcation_m2.cu.c:93-97).  Slow; for exact parity. */
+typedef enum sb_precision {
+    SB_PREC_FP32 = 0,
+    SB_PREC_SPLIT = 1,
+    SB_PREC_ORDERED = 2
+} sb_precision;
+
+/* Initial (x, y).
+ *   SB_INIT_UNIFORM   x, y ~ U[-0.1, 0.1) from a keyed hash of
+ *                     (seed, replica, spin, component); independent of batching.
+ *   SB_INIT_REFERENCE the reference's prepare_points (m2_opt.cu:48-56): one
+ *                     xorshift32 stream per spin seeded i + seed*N, replica r
+ *                     taking draws 2r+1, 2r+2; values in [0, 0.1). */
+typedef enum sb_init_kind { SB_INIT_UNIFORM = 0, SB_INIT_REFERENCE = 1 } sb_init_kind;
+
+/* Pump schedule: p_k = k / steps (m2_opt.cu:158); bSB/dSB use a(t) = a0 p_k. */
+typedef enum sb_schedule { SB_SCHED_LINEAR = 0 } sb_schedule;
+
+/* Storage format of J on the device (sb_step / sb_energy). */
+typedef enum sb_jfmt { SB_J_FP32 = 0, SB_J_BF16 = 1, SB_J_FP8 = 2 } sb_jfmt;
+
+/* Solver knobs.  The reference has them as compile-time defines
+ * (m2_opt.cu:10-22): steps <- STEP, dt <- DeltaT (coupling kick; the aSB local
+ * sub-step is dt/M, deltaT = 0.5 for DeltaT = 1, M = 2), a0 <- detune,
+ * c0 <- xi, K <- K, M <- M, replicas <- TIMES. */
+typedef struct sb_params {
+    int32_t variant;    /* sb_variant                                     */
+    int32_t precision;  /* sb_precision                                   */
+    int32_t init;       /* sb_init_kind                                   */
+    int32_t schedule;   /* sb_schedule                                    */
+    int32_t steps;      /* >= 1                                           */
+    int32_t replicas;   /* >= 1                                           */
+    int32_t M;          /* aSB local sub-steps per kick (>= 1)            */
+    int32_t reserved;
+    double dt;          /* time step                                      */
+    double a0;          /* detune / a0                                    */
+    double c0;          /* coupling; <= 0 selects the default (sb_c0_auto) */
+    double K;           /* aSB Kerr coefficient                           */
+    uint64_t seed;
+} sb_params;
+
+/* Outputs of sb_solve.  Pointer fields are caller-owned host buffers and
+ * may be NULL; the remaining fields are written by the library. */
+typedef struct sb_result {
+    int64_t *energy_i64;  /* [R] exact H when J is integer-valued             */
+    double *energy_f64;   /* [R] H in FP64 (exact for integer-valued J)       */
+    int8_t *spins;        /* [R*N] s = x > 0 ? +1 : -1, replica-major          */
+    float *x_final;       /* [R*N] final x                                    */
+    float *y_final;       /* [R*N] final y                                    */
+    int32_t best_replica; /* argmin of H (lowest index on ties)              */
+    int32_t energy_exact; /* 1 when energies are exact integers              */
+    double c0_used;
+    double step_ms;       /* device time of the step loop (HIP events)       */
+    double total_ms;      /* host wall time of the whole call                */
+} sb_result;
+
+typedef struct sb_ctx sb_ctx;
+typedef struct sb_problem sb_problem;
+
+/* ---- library ----------------------------------------------------------- */
+SB_API const char *sb_version(void);
+SB_API const char *sb_status_string(int status);
+/* Detail of the last failure on the calling thread ("" if none). */
+SB_API const char *sb_last_error_message(void);
+/* Reference defaults: aSB, steps 100, dt 1.0, a0 1, c0 auto, K 1, M 2,
+ * 1 replica, FP32, SB_INIT_REFERENCE, seed 0 (m2_opt.cu:10-22). */
+SB_API int sb_default_params(sb_params *p);
+
+/* ---- host-only instance helpers (no GPU needed) ------------------------ */
+/* Parse an edge list (m2_opt.cu:111-123): first integer N, then "i j w"
+ * triples; J[i][j] = J[j][i] = w (later lines overwrite).  Pass J = NULL to
+ * read N only.  J is written densely as [N][ldj].  Weights may be real. */
+SB_API int sb_edgelist_parse(const char *text, size_t len, int one_based, int32_t *N,
+                             float *J, int32_t ldj);
+/* sigma_J, integer flag and max |J_ij| of a dense symmetric J. */
+SB_API int sb_instance_stats(int32_t N, const float *J, int32_t ldj, double *sigma,
+                             int32_t *is_integer, double *max_abs);
+/* Default c0: bSB/dSB 0.5/(sqrt(N) sigma_J) (Goto 2021); aSB
+ * 0.7 a0/(sigma_J sqrt(N)) (m2_opt.cu:18 comment). */
+SB_API double sb_c0_auto(int32_t variant, int32_t N, double a0, double sigma);
+/* Seeded synthetic instances, dense [N][N] symmetric, zero diagonal:
+ *   kind 0: Erdos-Renyi G(N, p), weights +-1 equiprobable, J = -W (max-cut)
+ *   kind 1: Sherrington-Kirkpatrick, J_ij ~ N(0,1)/sqrt(N)
+ *   kind 2: complete graph, J_ij = +-1 equiprobable
+ *   kind 3: SK (N(0,1)/sqrt(N)) kept with probability p, else 0 */
+SB_API int sb_generate(int32_t kind, int32_t N, double p, uint64_t seed, float *J);
+
+/* ---- context and problems ---------------------------------------------- */
+SB_API int sb_ctx_create(int32_t device, sb_ctx **out);
+SB_API int sb_ctx_destroy(sb_ctx *ctx);
+/* Upload a dense host J (FP32, [N][ldj]); stores FP32 and, when exact, FP8
+ * (integer |J| <= 16) else BF16 forms on the device. */
+SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, int32_t ldj,
+                             sb_problem **out);
+SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *path, int one_based,
+                                         sb_problem **out);
+SB_API int sb_problem_destroy(sb_problem *prob);
+SB_API int sb_problem_info(const sb_problem *prob, int32_t *N, int32_t *is_integer,
+                           double *sigma, int32_t *split_jfmt);
+
+/* ---- solve (replaces the TIMES x STEP loop, m2_opt.cu:152-202) --------- */
+SB_API int sb_solve(sb_ctx *ctx, sb_problem *prob, const sb_params *params,
+                    sb_result *result);
+
+/* ---- low-level operator (replaces UpdateTwice + cublasSgemv) ----------- */
+/* Coefficients of one fused step (see sb_step_coeffs_for). */
+typedef struct sb_step_coeffs {
+    int32_t variant;
+    int32_t do_local; /* aSB: apply the next step's local sub-steps after the kick */
+    int32_t M;
+    int32_t reserved;
+    float alpha, beta, gamma, p_next;
+    double a0, K, dts;
+} sb_step_coeffs;
+
+/* Fill the coefficients of fused step k of an S-step schedule. */
+SB_API int sb_step_coeffs_for(const sb_params *p, double c0, int32_t k,
+                              sb_step_coeffs *out);
+
+/* One fused step on device buffers: F = J.X (or J.sign(X) for dSB), then the
+ * y/x update, wall and (aSB) local sub-steps; reads X, writes Xn, updates Y
+ * in place.  X, Xn, Y are FP32 [R][ldx]; J is [N][ldj] in jfmt; ldj, ldx are
+ * multiples of 64 and >= N; spins N..ldx-1 of X must be 0. */
+SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, int32_t jfmt,
+                   int32_t N, int32_t ldj, const float *dX, float *dXn, float *dY,
+                   int32_t R, int32_t ldx, const sb_step_coeffs *c, void *stream);
+
+/* Initial state on device buffers ([R][ldx], padding zeroed). */
+SB_API int sb_init_state(sb_ctx *ctx, int32_t init, uint64_t seed, int32_t N, int32_t R,
+                         int32_t ldx, float *dX, float *dY, void *stream);
+/* aSB only: the local sub-steps of step 0, applied before the first sb_step. */
+SB_API int sb_asb_prologue(sb_ctx *ctx, const sb_params *p, int32_t N, int32_t R,
+                           int32_t ldx, float *dX, float *dY, void *stream);
+
+/* H_r = -sum_{i<j} J_ij s_i s_j for s = sign(X) on device buffers; dH is a
+ * device FP64 array [R].  Exact for integer-valued J (|sum_j J_ij| < 2^52). */
+SB_API int sb_energy(sb_ctx *ctx, const float *dJ, int32_t N, int32_t ldj,
+                     const float *dX, int32_t R, int32_t ldx, double *dH, void *stream);
+
+/* ---- batched solve of many instances (one launch per step) ------------- */
+typedef struct sb_batch_stats {
+    int32_t work_items;   /* tiles per step across all instances */
+    int32_t workgroups;   /* persistent grid size                */
+    int32_t graph_used;   /* 1 if the step loop was a replayed hipGraph */
+    int32_t reserved;
+    double step_ms;       /* device time of the step loop        */
+    double total_ms;
+} sb_batch_stats;
+
+/* Solve n independent problems; params[i] / results[i] per problem.  All
+ * problems must share precision; steps, N, R, variant may differ.  Each
+ * problem's result is identical to sb_solve on it alone. */
+SB_API int sb_batch_solve(sb_ctx *ctx, int32_t n, sb_problem *const *probs,
+                          const sb_params *params, sb_result *results,
+                          sb_batch_stats *stats);
+
+/* ---- serving (continuous batching) -------------------------------------- */
+typedef struct sb_server sb_server;
+
+typedef struct sb_server_opts {
+    size_t hbm_budget_bytes;  /* device bytes for J + state of admitted requests */
+    int32_t chunk_steps;      /* steps run between admission points (>= 1)     */
+    int32_t max_active;       /* cap on concurrently resident requests          */
+    int32_t precision;        /* sb_precision for all requests                   */
+    int32_t reserved;
+} sb_server_opts;
+
+typedef struct sb_server_stats {
+    int64_t submitted, completed, rejected, cancelled;
+    int64_t active, queued;
+    size_t bytes_in_use;
+    double busy_ms;           /* device time spent in step launches */
+} sb_server_stats;
+
+SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, sb_server **out);
+/* Copies J (dense FP32 host [N][ldj]) and params; returns a ticket.  Rejects
+ * with SB_E_BUDGET when the request alone exceeds the budget. */
+SB_API int sb_server_submit(sb_server *srv, int32_t N, const float *J, int32_t ldj,
+                            const sb_params *params, uint64_t *ticket);
+/* Wait for a ticket (timeout_ms < 0: forever).  Fills result like sb_solve
+ * (spins/energies buffers sized for that request) and frees the ticket. */
+SB_API int sb_server_wait(sb_server *srv, uint64_t ticket, sb_result *result,
+                          double timeout_ms);
+SB_API int sb_server_cancel(sb_server *srv, uint64_t ticket);
+SB_API int sb_server_stats_get(sb_server *srv, sb_server_stats *out);
+SB_API int sb_server_destroy(sb_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

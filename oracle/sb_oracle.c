@@ -1,0 +1,523 @@
+/*
+ * sb_oracle.c -- CPU restatement of the SB solver loop.  TEST INFRASTRUCTURE:
+ * the checker for the HIP path, never part of it (see sb_oracle.h).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -mfma -fopenmp).  Every
+ * floating-point expression below is written op by op; contraction is off so
+ * that only the explicit fmaf/fma calls fuse.
+ */
+#include "sb_oracle.h"
+
+#include <ctype.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- RNG and init ------------------------------------------------------ */
+
+/* SB/CUDA_GPU/m2_opt.cu:38-46 */
+uint32_t orc_xorshift32(uint32_t *state)
+{
+    uint32_t v = *state;
+    v ^= v << 13;
+    v ^= v >> 17;
+    v ^= v << 5;
+    *state = v;
+    return v;
+}
+
+/* SB/CUDA_GPU/m2_opt.cu:54-55: (xs / (float)MAX) / 10.0, stored as float.
+ * (float)4294967295.0 rounds to 2^32, so the FP32 division is the exact
+ * scaling (float)v * 2^-32 (also under the reference's fast-math division,
+ * whose reciprocal of a power of two is exact). */
+float orc_ref_draw_value(uint32_t v)
+{
+    float q = (float)v / 4294967296.0f;
+    double d = (double)q / 10.0;
+    return (float)d;
+}
+
+void orc_init_reference(int N, int R, int ld, uint32_t seed, float *x, float *y)
+{
+    for (int r = 0; r < R; ++r)
+        for (int i = N; i < ld; ++i) {
+            x[(size_t)r * ld + i] = 0.0f;
+            y[(size_t)r * ld + i] = 0.0f;
+        }
+    for (int i = 0; i < N; ++i) {
+        /* m2_opt.cu:136: initRand[i] = i; state carries over trials (:152). */
+        uint32_t st = (uint32_t)i + seed * (uint32_t)N;
+        for (int r = 0; r < R; ++r) {
+            x[(size_t)r * ld + i] = orc_ref_draw_value(orc_xorshift32(&st));
+            y[(size_t)r * ld + i] = orc_ref_draw_value(orc_xorshift32(&st));
+        }
+    }
+}
+
+uint32_t orc_keyed_u24(uint64_t seed, uint32_t replica, uint32_t spin, uint32_t comp)
+{
+    uint64_t key = ((uint64_t)replica << 32) | (uint64_t)spin;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (2ull * key + (uint64_t)comp + 1ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (uint32_t)(z >> 40);
+}
+
+static float keyed_value(uint32_t u24)
+{
+    float v = (float)u24 * 0x1p-23f; /* exact, in [0, 2) */
+    v = v - 1.0f;                    /* exact, in [-1, 1) */
+    return v * 0.1f;                 /* one rounding */
+}
+
+void orc_init_uniform(int N, int R, int ld, uint64_t seed, float *x, float *y)
+{
+    for (int r = 0; r < R; ++r) {
+        for (int i = 0; i < ld; ++i) {
+            size_t o = (size_t)r * ld + i;
+            if (i < N) {
+                x[o] = keyed_value(orc_keyed_u24(seed, (uint32_t)r, (uint32_t)i, 0));
+                y[o] = keyed_value(orc_keyed_u24(seed, (uint32_t)r, (uint32_t)i, 1));
+            } else {
+                x[o] = 0.0f;
+                y[o] = 0.0f;
+            }
+        }
+    }
+}
+
+/* ---- parameters -------------------------------------------------------- */
+
+double orc_sigma(int N, const float *J, int ldj)
+{
+    if (N < 2) return 0.0;
+    double s = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            if (i != j) {
+                double v = (double)J[(size_t)i * ldj + j];
+                s += v * v;
+            }
+    return sqrt(s / ((double)N * (double)(N - 1)));
+}
+
+double orc_c0_auto(int variant, int N, double a0, const float *J, int ldj)
+{
+    double sg = orc_sigma(N, J, ldj);
+    if (sg <= 0.0) return 0.0;
+    if (variant == ORC_ASB) return 0.7 * a0 / (sg * sqrt((double)N));
+    return 0.5 / (sqrt((double)N) * sg);
+}
+
+void orc_coeffs(int variant, int k, int steps, double dt, double a0, double c0,
+                float out[4])
+{
+    if (variant == ORC_ASB) {
+        float p = (float)k / (float)steps; /* m2_opt.cu:158 */
+        out[0] = 0.0f;
+        out[1] = (float)(dt * c0); /* m2_opt.cu:148: coeff1 = DeltaT * xi */
+        out[2] = 0.0f;
+        out[3] = p;
+        return;
+    }
+    double p = (double)k / (double)steps;
+    double a = a0 * p;
+    out[0] = (float)(dt * (a0 - a));
+    out[1] = (float)(dt * c0);
+    out[2] = (float)(dt * a0);
+    out[3] = (float)p;
+}
+
+/* ---- field ------------------------------------------------------------- */
+
+static inline float sgnf(float v) { return v > 0.0f ? 1.0f : -1.0f; }
+
+/* Rows are processed eight at a time so the sequential fmaf chains of
+ * different rows overlap in the CPU pipeline; each row's own order is the
+ * plain j = 0..N-1 chain of Bifurcation_m2.cu.c:93-97. */
+static void field_rows_f32(int N, const float *J, int ldj, const float *xs, float *F)
+{
+    int i = 0;
+    for (; i + 8 <= N; i += 8) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f, a6 = 0.f, a7 = 0.f;
+        const float *r0 = J + (size_t)i * ldj;
+        for (int j = 0; j < N; ++j) {
+            float v = xs[j];
+            a0 = fmaf(r0[j], v, a0);
+            a1 = fmaf(r0[(size_t)ldj + j], v, a1);
+            a2 = fmaf(r0[2 * (size_t)ldj + j], v, a2);
+            a3 = fmaf(r0[3 * (size_t)ldj + j], v, a3);
+            a4 = fmaf(r0[4 * (size_t)ldj + j], v, a4);
+            a5 = fmaf(r0[5 * (size_t)ldj + j], v, a5);
+            a6 = fmaf(r0[6 * (size_t)ldj + j], v, a6);
+            a7 = fmaf(r0[7 * (size_t)ldj + j], v, a7);
+        }
+        F[i] = a0; F[i + 1] = a1; F[i + 2] = a2; F[i + 3] = a3;
+        F[i + 4] = a4; F[i + 5] = a5; F[i + 6] = a6; F[i + 7] = a7;
+    }
+    for (; i < N; ++i) {
+        float a = 0.f;
+        const float *row = J + (size_t)i * ldj;
+        for (int j = 0; j < N; ++j) a = fmaf(row[j], xs[j], a);
+        F[i] = a;
+    }
+}
+
+void orc_field_f32(int N, int R, const float *J, int ldj, const float *x, int ld,
+                   int use_sign, float *F)
+{
+    float *s = use_sign ? (float *)malloc(sizeof(float) * (size_t)(N > 0 ? N : 1)) : NULL;
+    for (int r = 0; r < R; ++r) {
+        const float *xr = x + (size_t)r * ld;
+        const float *src = xr;
+        if (use_sign) {
+            for (int j = 0; j < N; ++j) s[j] = sgnf(xr[j]);
+            src = s;
+        }
+        field_rows_f32(N, J, ldj, src, F + (size_t)r * ld);
+    }
+    free(s);
+}
+
+void orc_field_f64(int N, int R, const float *J, int ldj, const double *x, int ld,
+                   int use_sign, double *F)
+{
+    for (int r = 0; r < R; ++r) {
+        const double *xr = x + (size_t)r * ld;
+        for (int i = 0; i < N; ++i) {
+            const float *row = J + (size_t)i * ldj;
+            double a = 0.0;
+            for (int j = 0; j < N; ++j) {
+                double v = use_sign ? (xr[j] > 0.0 ? 1.0 : -1.0) : xr[j];
+                a = fma((double)row[j], v, a);
+            }
+            F[(size_t)r * ld + i] = a;
+        }
+    }
+}
+
+/* ---- updates ----------------------------------------------------------- */
+
+/* bSB / dSB, Goto et al. 2021: y += (-(a0 - a) x + c0 F) dt; x += a0 y dt;
+ * inelastic wall at |x| = 1.  With alpha = dt (a0 - a), beta = dt c0,
+ * gamma = dt a0 this is the op sequence below. */
+void orc_bsb_update_f32(float F, float alpha, float beta, float gamma, float *x,
+                        float *y)
+{
+    float yn = fmaf(beta, F, fmaf(-alpha, *x, *y));
+    float xn = fmaf(gamma, yn, *x);
+    if (fabsf(xn) > 1.0f) {
+        xn = copysignf(1.0f, xn);
+        yn = 0.0f;
+    }
+    *x = xn;
+    *y = yn;
+}
+
+static void bsb_update_f64(double F, double alpha, double beta, double gamma,
+                           double *x, double *y)
+{
+    double yn = fma(beta, F, fma(-alpha, *x, *y));
+    double xn = fma(gamma, yn, *x);
+    if (fabs(xn) > 1.0) {
+        xn = copysign(1.0, xn);
+        yn = 0.0;
+    }
+    *x = xn;
+    *y = yn;
+}
+
+/* SB/CUDA_GPU/m2_opt.cu:58-75.  detune, deltaT and K are double literals in
+ * the reference, so each statement is evaluated in FP64 from the FP32
+ * operands and rounded to FP32 on assignment:
+ *   lx = lx + detune * ly * deltaT;
+ *   ly = ly - deltaT * (K * lx * lx * lx + (detune - amplitude) * lx); */
+void orc_asb_local_f32(float *x, float *y, float p, double a0, double K,
+                       double dts, int M)
+{
+    float lx = *x, ly = *y;
+    for (int m = 0; m < M; ++m) {
+        double t = a0 * (double)ly;
+        t = t * dts;
+        lx = (float)((double)lx + t);
+        double xd = (double)lx;
+        double cube = K * xd;
+        cube = cube * xd;
+        cube = cube * xd;
+        double lin = a0 - (double)p;
+        lin = lin * xd;
+        double u = cube + lin;
+        u = dts * u;
+        ly = (float)((double)ly - u);
+    }
+    *x = lx;
+    *y = ly;
+}
+
+static void asb_local_f64(double *x, double *y, double p, double a0, double K,
+                          double dts, int M)
+{
+    double lx = *x, ly = *y;
+    for (int m = 0; m < M; ++m) {
+        lx = lx + (a0 * ly) * dts;
+        ly = ly - dts * (((K * lx) * lx) * lx + (a0 - p) * lx);
+    }
+    *x = lx;
+    *y = ly;
+}
+
+/* ---- solver loops ------------------------------------------------------ */
+
+static int run_f32_replica(int variant, int N, const float *J, int ldj, float *x,
+                           float *y, float *F, float *s, int k0, int k1, int steps,
+                           double dt, double a0, double c0, double K, int M)
+{
+    double dts = dt / (double)M;
+    for (int k = k0; k < k1; ++k) {
+        float cf[4];
+        orc_coeffs(variant, k, steps, dt, a0, c0, cf);
+        if (variant == ORC_ASB) {
+            /* m2_opt.cu:168-169: local sub-steps, then the kick with new x */
+            for (int i = 0; i < N; ++i) orc_asb_local_f32(&x[i], &y[i], cf[3], a0, K, dts, M);
+            field_rows_f32(N, J, ldj, x, F);
+            for (int i = 0; i < N; ++i) y[i] = fmaf(cf[1], F[i], y[i]);
+        } else {
+            const float *src = x;
+            if (variant == ORC_DSB) {
+                for (int j = 0; j < N; ++j) s[j] = sgnf(x[j]);
+                src = s;
+            }
+            field_rows_f32(N, J, ldj, src, F);
+            for (int i = 0; i < N; ++i) orc_bsb_update_f32(F[i], cf[0], cf[1], cf[2], &x[i], &y[i]);
+        }
+    }
+    return ORC_OK;
+}
+
+static int check_run_args(int variant, int N, int R, int ldj, int ld, int k0, int k1,
+                          int steps, int M)
+{
+    if (variant < ORC_ASB || variant > ORC_DSB) return ORC_E_INVALID;
+    if (N < 1 || R < 0 || ldj < N || ld < N || steps < 1) return ORC_E_INVALID;
+    if (k0 < 0 || k1 < k0 || k1 > steps) return ORC_E_INVALID;
+    if (variant == ORC_ASB && M < 1) return ORC_E_INVALID;
+    return ORC_OK;
+}
+
+int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
+                float *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                double c0, double K, int M, int nthreads)
+{
+    int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
+    if (rc) return rc;
+    if (!orc_has_fma()) return ORC_E_NOFMA;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        float *F = (float *)malloc(sizeof(float) * (size_t)N);
+        float *s = (float *)malloc(sizeof(float) * (size_t)N);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int r = 0; r < R; ++r)
+            run_f32_replica(variant, N, J, ldj, x + (size_t)r * ld, y + (size_t)r * ld,
+                            F, s, k0, k1, steps, dt, a0, c0, K, M);
+        free(F);
+        free(s);
+    }
+    return ORC_OK;
+}
+
+int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
+                double *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                double c0, double K, int M, int nthreads)
+{
+    int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
+    if (rc) return rc;
+    double dts = dt / (double)M;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        double *F = (double *)malloc(sizeof(double) * (size_t)N);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int r = 0; r < R; ++r) {
+            double *xr = x + (size_t)r * ld, *yr = y + (size_t)r * ld;
+            for (int k = k0; k < k1; ++k) {
+                double p = (double)k / (double)steps;
+                if (variant == ORC_ASB) {
+                    for (int i = 0; i < N; ++i) asb_local_f64(&xr[i], &yr[i], p, a0, K, dts, M);
+                    orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
+                    for (int i = 0; i < N; ++i) yr[i] = fma(dt * c0, F[i], yr[i]);
+                } else {
+                    orc_field_f64(N, 1, J, ldj, xr, ld, variant == ORC_DSB, F);
+                    double alpha = dt * (a0 - a0 * p), beta = dt * c0, gamma = dt * a0;
+                    for (int i = 0; i < N; ++i) bsb_update_f64(F[i], alpha, beta, gamma, &xr[i], &yr[i]);
+                }
+            }
+        }
+        free(F);
+    }
+    return ORC_OK;
+}
+
+/* ---- readout and energy ------------------------------------------------ */
+
+void orc_spins_f32(int N, int R, const float *x, int ld, int8_t *s)
+{
+    for (int r = 0; r < R; ++r)
+        for (int i = 0; i < N; ++i)
+            s[(size_t)r * N + i] = x[(size_t)r * ld + i] > 0.0f ? 1 : -1;
+}
+
+/* SB/CUDA_GPU/m2_opt.cu:192-202 with an int64 accumulator. */
+int orc_energy_i64(int N, int R, const float *J, int ldj, const int8_t *s, int64_t *H)
+{
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            float v = J[(size_t)i * ldj + j];
+            if (v != floorf(v) || fabsf(v) > 16777216.0f) return ORC_E_INVALID;
+        }
+    for (int r = 0; r < R; ++r) {
+        const int8_t *sr = s + (size_t)r * N;
+        int64_t E = 0;
+        for (int i = 0; i < N; ++i)
+            for (int j = i + 1; j < N; ++j)
+                E += (int64_t)sr[i] * (int64_t)sr[j] * (int64_t)J[(size_t)i * ldj + j];
+        H[r] = -E;
+    }
+    return ORC_OK;
+}
+
+void orc_energy_f64(int N, int R, const float *J, int ldj, const int8_t *s, double *H)
+{
+    for (int r = 0; r < R; ++r) {
+        const int8_t *sr = s + (size_t)r * N;
+        double E = 0.0;
+        for (int i = 0; i < N; ++i)
+            for (int j = i + 1; j < N; ++j)
+                E += (double)(sr[i] * sr[j]) * (double)J[(size_t)i * ldj + j];
+        H[r] = -E;
+    }
+}
+
+int orc_ground_state(int N, const float *J, int ldj, double *H, int8_t *s)
+{
+    if (N < 1 || N > 24) return ORC_E_INVALID;
+    double best = INFINITY;
+    uint32_t best_m = 0;
+    for (uint32_t m = 0; m < (1u << N); ++m) {
+        double E = 0.0;
+        for (int i = 0; i < N; ++i) {
+            int si = (m >> i) & 1 ? 1 : -1;
+            for (int j = i + 1; j < N; ++j) {
+                int sj = (m >> j) & 1 ? 1 : -1;
+                E += (double)(si * sj) * (double)J[(size_t)i * ldj + j];
+            }
+        }
+        if (-E < best) {
+            best = -E;
+            best_m = m;
+        }
+    }
+    *H = best;
+    for (int i = 0; i < N; ++i) s[i] = (best_m >> i) & 1 ? 1 : -1;
+    return ORC_OK;
+}
+
+/* ---- edge list --------------------------------------------------------- */
+
+typedef struct {
+    const char *p, *end;
+} cursor;
+
+static void skip_ws(cursor *c)
+{
+    while (c->p < c->end && isspace((unsigned char)*c->p)) ++c->p;
+}
+
+static int read_long(cursor *c, long *out)
+{
+    skip_ws(c);
+    if (c->p >= c->end) return 0;
+    char buf[32];
+    size_t n = 0;
+    while (c->p < c->end && n < sizeof(buf) - 1 && !isspace((unsigned char)*c->p))
+        buf[n++] = *c->p++;
+    buf[n] = 0;
+    char *e = NULL;
+    long v = strtol(buf, &e, 10);
+    if (e == buf || *e) return -1;
+    *out = v;
+    return 1;
+}
+
+static int read_double(cursor *c, double *out)
+{
+    skip_ws(c);
+    if (c->p >= c->end) return 0;
+    char buf[64];
+    size_t n = 0;
+    while (c->p < c->end && n < sizeof(buf) - 1 && !isspace((unsigned char)*c->p))
+        buf[n++] = *c->p++;
+    buf[n] = 0;
+    char *e = NULL;
+    double v = strtod(buf, &e);
+    if (e == buf || *e || !isfinite(v)) return -1;
+    *out = v;
+    return 1;
+}
+
+int orc_edgelist_header(const char *text, size_t len, int *N)
+{
+    cursor c = {text, text + len};
+    long n;
+    if (read_long(&c, &n) != 1 || n < 1 || n > (1L << 30)) return ORC_E_PARSE;
+    *N = (int)n;
+    return ORC_OK;
+}
+
+int orc_edgelist_fill(const char *text, size_t len, int one_based, int N, float *J,
+                      int ldj)
+{
+    cursor c = {text, text + len};
+    long n;
+    if (read_long(&c, &n) != 1 || n != N) return ORC_E_PARSE;
+    for (int i = 0; i < N; ++i) memset(J + (size_t)i * ldj, 0, sizeof(float) * (size_t)N);
+    for (;;) {
+        long a, b;
+        double w;
+        int ra = read_long(&c, &a);
+        if (ra == 0) break;
+        if (ra < 0 || read_long(&c, &b) != 1 || read_double(&c, &w) != 1) return ORC_E_PARSE;
+        if (one_based) {
+            a -= 1;
+            b -= 1;
+        }
+        if (a < 0 || b < 0 || a >= N || b >= N) return ORC_E_RANGE;
+        if (a == b) return ORC_E_INVALID;
+        J[(size_t)a * ldj + b] = (float)w;
+        J[(size_t)b * ldj + a] = (float)w;
+    }
+    return ORC_OK;
+}
+
+int orc_has_fma(void)
+{
+#if defined(__x86_64__) || defined(__i386__)
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("fma") ? 1 : 0;
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,136 @@
+/*
+ * sb_oracle.h -- CPU restatement of the simulated-bifurcation (SB) solver loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it.  Nothing in annealing_amd/ links or calls it.
+ *
+ * What it restates (paths relative to the reference repository root):
+ *   - xorshift32                  SB/CUDA_GPU/m2_opt.cu:38-46
+ *   - prepare_points init         SB/CUDA_GPU/m2_opt.cu:48-56, seeds :133-138
+ *   - UpdateTwice (aSB local)     SB/CUDA_GPU/m2_opt.cu:58-75 (FP64-evaluated,
+ *                                 FP32-stored, constants :16-21)
+ *   - coupling kick               SB/CUDA_GPU/m2_opt.cu:148-149,169 (SGEMV
+ *                                 y <- (DeltaT*xi) J x + y); summation order
+ *                                 of the naive twin Bifurcation_m2.cu.c:93-97
+ *                                 (sequential fmaf over j)
+ *   - pump schedule               SB/CUDA_GPU/m2_opt.cu:158 (s/(float)STEP)
+ *   - sign readout + energy       SB/CUDA_GPU/m2_opt.cu:192-202 (x>0 ? +1 : -1,
+ *                                 H = -sum_{i<j} J_ij s_i s_j), int64 here
+ *   - edge-list loader            SB/CUDA_GPU/m2_opt.cu:111-123
+ *   - bSB / dSB                   not in the reference; Goto et al.,
+ *                                 Sci. Adv. 7:eabe7953 (2021), symplectic
+ *                                 Euler (y first, then x) with inelastic walls
+ *
+ * Layout of replica state: x[r*ld + i], y[r*ld + i] (replica-major, spin
+ * contiguous), J[i*ldj + j] row-major, dense, symmetric, zero diagonal.
+ */
+#ifndef SB_ORACLE_H
+#define SB_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_ASB = 0, ORC_BSB = 1, ORC_DSB = 2 };
+
+enum {
+    ORC_OK = 0,
+    ORC_E_INVALID = 1,
+    ORC_E_PARSE = 3,
+    ORC_E_RANGE = 4,
+    ORC_E_NOFMA = 9
+};
+
+/* Marsaglia xorshift32 (13, 17, 5); SB/CUDA_GPU/m2_opt.cu:38-46. */
+uint32_t orc_xorshift32(uint32_t *state);
+
+/* Value prepare_points stores for one draw v (m2_opt.cu:54-55):
+ * ((float)v / (float)4294967295.0) is an FP32 division (exact here: the
+ * divisor rounds to 2^32), then "/ 10.0" in FP64, stored as FP32. */
+float orc_ref_draw_value(uint32_t v);
+
+/* Reference init: spin i of the state stream seeded (uint32)(i + seed*N)
+ * (seed 0 is the reference's st_i = i).  Replica r is trial r of the
+ * reference TIMES loop: it takes draws 2r+1 (x) and 2r+2 (y) of spin i's
+ * stream.  Spins N..ld-1 are set to 0. */
+void orc_init_reference(int N, int R, int ld, uint32_t seed, float *x, float *y);
+
+/* Keyed uniform init x, y ~ U[-0.1, 0.1): one splitmix64-style hash of
+ * (seed, replica, spin, component) per value; see orc_keyed_u24. */
+uint32_t orc_keyed_u24(uint64_t seed, uint32_t replica, uint32_t spin, uint32_t comp);
+void orc_init_uniform(int N, int R, int ld, uint64_t seed, float *x, float *y);
+
+/* sigma_J = sqrt(sum_{i!=j} J^2 / (N(N-1))) and the Goto-2021 default
+ * c0 = 0.5 / (sqrt(N) sigma_J) for bSB/dSB; the reference's aSB default
+ * xi = 0.7 a0 / (sigma_J sqrt(N)) (m2_opt.cu:18 comment). */
+double orc_sigma(int N, const float *J, int ldj);
+double orc_c0_auto(int variant, int N, double a0, const float *J, int ldj);
+
+/* Per-step coefficients (single source of the definition):
+ *   bSB/dSB: p = k/S (double), a = a0 p, alpha = fl32(dt (a0 - a)),
+ *            beta = fl32(dt c0), gamma = fl32(dt a0)
+ *   aSB:     p = fl32(k)/fl32(S) in FP32 (m2_opt.cu:158), kick = fl32(dt c0)
+ * out[0..3] = alpha, beta, gamma, p */
+void orc_coeffs(int variant, int k, int steps, double dt, double a0, double c0,
+                float out[4]);
+
+/* One field evaluation per replica in FP32, sequential fmaf over j
+ * (Bifurcation_m2.cu.c:93-97 order).  dSB uses s_j = x_j > 0 ? 1 : -1. */
+void orc_field_f32(int N, int R, const float *J, int ldj, const float *x, int ld,
+                   int use_sign, float *F);
+void orc_field_f64(int N, int R, const float *J, int ldj, const double *x, int ld,
+                   int use_sign, double *F);
+
+/* FP32 epilogues, exactly as the product defines them. */
+void orc_bsb_update_f32(float F, float alpha, float beta, float gamma,
+                        float *x, float *y);
+void orc_asb_local_f32(float *x, float *y, float p, double a0, double K,
+                       double dts, int M);
+
+/* Run steps [k0, k1) of an S-step schedule on R replicas in FP32 (the plain
+ * path's definition).  x, y are in/out [R][ld].  For aSB the state between
+ * calls is the reference's (x, y) after the kick of the last step run.
+ * nthreads > 1 splits replicas over OpenMP threads (same results). */
+int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
+                float *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                double c0, double K, int M, int nthreads);
+
+/* Same schedule in FP64 (the FP64 oracle trajectory). */
+int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
+                double *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                double c0, double K, int M, int nthreads);
+
+/* Sign readout s_i = x_i > 0 ? +1 : -1 (m2_opt.cu:196-197). */
+void orc_spins_f32(int N, int R, const float *x, int ld, int8_t *s);
+
+/* Exact energy H = -sum_{i<j} J_ij s_i s_j for integer-valued J (int64);
+ * returns ORC_E_INVALID if some J_ij is not an integer. */
+int orc_energy_i64(int N, int R, const float *J, int ldj, const int8_t *s,
+                   int64_t *H);
+/* FP64 energy for real J, sequential i<j order. */
+void orc_energy_f64(int N, int R, const float *J, int ldj, const int8_t *s,
+                    double *H);
+
+/* Exhaustive ground state for N <= 24 (brute force over 2^N states). */
+int orc_ground_state(int N, const float *J, int ldj, double *H, int8_t *s);
+
+/* Edge list (m2_opt.cu:111-123): first integer N, then "i j w" triples until
+ * end of input, J[i][j] = J[j][i] = w, later triples overwrite earlier ones.
+ * Weights may be integer or real.  i == j and out-of-range indices are
+ * errors here (the reference asserts on the first and overwrites the heap on
+ * the second).  orc_edgelist_header reads N only. */
+int orc_edgelist_header(const char *text, size_t len, int *N);
+int orc_edgelist_fill(const char *text, size_t len, int one_based, int N,
+                      float *J, int ldj);
+
+/* 1 when the host CPU has FMA (the FP32 restatement needs exact fmaf). */
+int orc_has_fma(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

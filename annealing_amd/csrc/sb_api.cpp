@@ -337,13 +337,13 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
 
 static void free_ws(sb_ctx *c)
 {
-    hipFree(c->X0);
-    hipFree(c->X1);
-    hipFree(c->Y);
-    hipFree(c->H);
-    hipFree(c->part);
-    hipFree(c->S);
-    hipFree(c->tmp);
+    (void)hipFree(c->X0);
+    (void)hipFree(c->X1);
+    (void)hipFree(c->Y);
+    (void)hipFree(c->H);
+    (void)hipFree(c->part);
+    (void)hipFree(c->S);
+    (void)hipFree(c->tmp);
     c->X0 = c->X1 = c->Y = nullptr;
     c->H = c->part = nullptr;
     c->S = nullptr;
@@ -354,12 +354,19 @@ static void free_ws(sb_ctx *c)
 extern "C" SB_API int sb_ctx_destroy(sb_ctx *ctx)
 {
     if (!ctx) return SB_E_INVALID;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        if (ctx->live_problems > 0) {
+            sbh::set_error("sb_ctx_destroy: destroy the context's problems first");
+            return SB_E_INVALID;
+        }
+    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     free_ws(ctx);
-    hipEventDestroy(ctx->ev0);
-    hipEventDestroy(ctx->ev1);
-    hipStreamDestroy(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SB_OK;
 }
@@ -386,7 +393,7 @@ int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N)
 {
     int rc;
     if (ctx->state_cap < state_elems) {
-        hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
         size_t c0 = ctx->state_cap, c1 = ctx->state_cap, c2 = ctx->state_cap;
         if ((rc = grow(&ctx->X0, &c0, state_elems))) return rc;
         if ((rc = grow(&ctx->X1, &c1, state_elems))) return rc;
@@ -446,12 +453,13 @@ extern "C" SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, 
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-        hipFree(p->J32);
-        hipFree(p->Jbf16);
-        hipFree(p->Jfp8);
+        (void)hipFree(p->J32);
+        (void)hipFree(p->Jbf16);
+        (void)hipFree(p->Jfp8);
         delete p;
         return sbh::hip_fail(e, "sb_problem_create");
     }
+    ctx->live_problems++;
     *out = p;
     return SB_OK;
 }
@@ -485,11 +493,12 @@ extern "C" SB_API int sb_problem_destroy(sb_problem *p)
 {
     if (!p) return SB_E_INVALID;
     std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
-    hipSetDevice(p->ctx->device);
-    hipStreamSynchronize(p->ctx->stream);
-    hipFree(p->J32);
-    hipFree(p->Jbf16);
-    hipFree(p->Jfp8);
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    (void)hipFree(p->J32);
+    (void)hipFree(p->Jbf16);
+    (void)hipFree(p->Jfp8);
+    p->ctx->live_problems--;
     delete p;
     return SB_OK;
 }

@@ -26,6 +26,7 @@ struct sb_ctx {
     // Autotuned (family, shape) per (family set, N bucket, R bucket).
     std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tune;
     bool autotune = true;
+    int live_problems = 0;  // guarded by mu
 };
 
 struct sb_problem {

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 from typing import Optional
 
@@ -266,6 +267,7 @@ class Problem:
     def __init__(self, ctx: "Context", handle: C.c_void_p):
         self.ctx = ctx
         self.handle = handle
+        ctx._problems.add(self)
         N, integer, sigma, jf = C.c_int32(), C.c_int32(), C.c_double(), C.c_int32()
         _check(load().sb_problem_info(handle, C.byref(N), C.byref(integer), C.byref(sigma), C.byref(jf)),
                "sb_problem_info")
@@ -275,9 +277,10 @@ class Problem:
         self.split_jfmt = jf.value
 
     def close(self) -> None:
-        if self.handle:
+        # A problem must be destroyed before its context.
+        if self.handle and self.ctx.handle:
             load().sb_problem_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -294,11 +297,20 @@ class Context:
         h = C.c_void_p()
         _check(self.lib.sb_ctx_create(device, C.byref(h)), "sb_ctx_create")
         self.handle = h
+        self._problems = weakref.WeakSet()
 
     def close(self) -> None:
         if self.handle:
+            for p in list(self._problems):
+                p.close()
             self.lib.sb_ctx_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

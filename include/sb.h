@@ -156,6 +156,7 @@ SB_API int sb_generate(int32_t kind, int32_t N, double p, uint64_t seed, float *
 
 /* ---- context and problems ---------------------------------------------- */
 SB_API int sb_ctx_create(int32_t device, sb_ctx **out);
+/* Fails with SB_E_INVALID while problems created on ctx are still alive. */
 SB_API int sb_ctx_destroy(sb_ctx *ctx);
 /* Upload a dense host J (FP32, [N][ldj]); stores FP32 and, when exact, FP8
  * (integer |J| <= 16) else BF16 forms on the device. */

@@ -63,7 +63,7 @@ def test_init_bit_exact(orc, init):
 @pytest.mark.parametrize("variant", ["asb", "bsb", "dsb"])
 @pytest.mark.parametrize("N,R,steps,init", [(67, 5, 120, "uniform"), (200, 12, 150, "reference")])
 def test_ordered_matches_oracle_bit_exact(ctx, orc, variant, N, R, steps, init):
-    J = sk(N, 11) if variant != "dsb" else kn_pm1(N, 12)
+    J = sk(N, 11) if variant == "bsb" else kn_pm1(N, 12)
     v = sb.VARIANTS[variant]
     dt = 1.0 if variant == "asb" else 0.5
     prob = ctx.problem(J)
@@ -71,8 +71,9 @@ def test_ordered_matches_oracle_bit_exact(ctx, orc, variant, N, R, steps, init):
                     init=init, seed=3, dt=dt)
     xo, yo, c0 = oracle_run(orc, v, J, R, steps, init, 3, dt)
     assert sol.c0 == pytest.approx(c0, rel=1e-12)
-    assert np.array_equal(sol.x, xo), np.max(np.abs(sol.x - xo))
-    assert np.array_equal(sol.y, yo)
+    # (aSB can run away to inf/NaN for some replicas; NaN == NaN here)
+    assert np.array_equal(sol.x, xo, equal_nan=True), np.nanmax(np.abs(sol.x - xo))
+    assert np.array_equal(sol.y, yo, equal_nan=True)
     s = orc.spins(np.ascontiguousarray(xo), N)
     assert np.array_equal(sol.spins, s)
     if prob.integer:

@@ -331,6 +331,8 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
     }
     const char *at = std::getenv("SB_AUTOTUNE");
     if (at && at[0] == '0') c->autotune = false;
+    const char *vb = std::getenv("SB_VERBOSE");
+    if (vb && vb[0] != '0') c->verbose = true;
     *out = c;
     return SB_OK;
 }
@@ -344,6 +346,10 @@ static void free_ws(sb_ctx *c)
     (void)hipFree(c->part);
     (void)hipFree(c->S);
     (void)hipFree(c->tmp);
+    (void)hipFree(c->P0);
+    (void)hipFree(c->P1);
+    c->P0 = c->P1 = nullptr;
+    c->plane_cap = 0;
     c->X0 = c->X1 = c->Y = nullptr;
     c->H = c->part = nullptr;
     c->S = nullptr;
@@ -399,6 +405,13 @@ int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N)
         if ((rc = grow(&ctx->X1, &c1, state_elems))) return rc;
         if ((rc = grow(&ctx->Y, &c2, state_elems))) return rc;
         ctx->state_cap = state_elems;
+    }
+    if (ctx->plane_cap < 4 * state_elems) {
+        (void)hipStreamSynchronize(ctx->stream);
+        size_t c0 = ctx->plane_cap, c1 = ctx->plane_cap;
+        if ((rc = grow(&ctx->P0, &c0, 4 * state_elems))) return rc;
+        if ((rc = grow(&ctx->P1, &c1, 4 * state_elems))) return rc;
+        ctx->plane_cap = 4 * state_elems;
     }
     if ((rc = grow(&ctx->H, &ctx->H_cap, (size_t)std::max(R, 1)))) return rc;
     if ((rc = grow(&ctx->part, &ctx->part_cap, std::max<size_t>(energy_partial_elems(N, R), 1)))) return rc;
@@ -614,11 +627,11 @@ int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int varian
         return SB_OK;
     }
     // Time each candidate for a few launches on the workspace (its contents
-    // are overwritten by the init that follows).
+    // are overwritten by the init that follows).  The state is random, not
+    // zero: MFMA timing depends on the operand values (power).
     int rc = ensure_workspace(ctx, (size_t)R * prob->ld, R, N);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(ctx->X0, 0, (size_t)R * prob->ld * sizeof(float), ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->Y, 0, (size_t)R * prob->ld * sizeof(float), ctx->stream));
+    HIP_TRY(launch_init(SB_INIT_UNIFORM, 0x5eed, N, R, prob->ld, ctx->X0, ctx->Y, ctx->X1, ctx->stream));
     StepArgs a{};
     a.X = ctx->X0;
     a.Xn = ctx->X1;
@@ -632,8 +645,13 @@ int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int varian
     float best_ms = 1e30f;
     int best_f = fams[0], best_s = 0;
     const int reps = 3;
+    a.bplane = (size_t)R * prob->ld;
     for (int f : fams) {
         a.J = family_J(f, prob);
+        a.bkind = plane_kind(f, variant);
+        a.B = a.bkind != BK_NONE ? (const void *)ctx->P0 : nullptr;
+        a.Bn = a.bkind != BK_NONE ? (void *)ctx->P1 : nullptr;
+        if (a.bkind != BK_NONE) HIP_TRY(launch_make_planes(a.bkind, ctx->X0, R, prob->ld, ctx->P0, ctx->stream));
         for (int s = 0; s < shape_count(f); ++s) {
             HIP_TRY(launch_step(f, s, a, ctx->stream));  // warm-up
             HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
@@ -642,6 +660,11 @@ int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int varian
             HIP_TRY(hipEventSynchronize(ctx->ev1));
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ctx->verbose) {
+                TileShape sh = shape_at(f, s);
+                std::fprintf(stderr, "[sb autotune] N=%d R=%d family=%d shape=%d (%dx%d, %d thr) %.2f us\n", N, R, f,
+                             s, sh.mt(), sh.nt(), sh.threads(), ms * 1e3f / reps);
+            }
             if (ms < best_ms) {
                 best_ms = ms;
                 best_f = f;
@@ -720,16 +743,25 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     a.ldx = ld;
     a.N = N;
     a.R = R;
+    a.bkind = p->precision == SB_PREC_ORDERED ? BK_NONE : plane_kind(family, p->variant);
+    a.bplane = (size_t)R * ld;
+    unsigned char *P = ctx->P0, *Pn = ctx->P1;
+    if (a.bkind != BK_NONE) HIP_TRY(launch_make_planes(a.bkind, X, R, ld, P, s));
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     for (int k = 0; k < p->steps; ++k) {
         sb_step_coeffs_for(p, c0, k, &a.c);
         a.X = X;
         a.Xn = Xn;
         a.Y = Y;
+        if (a.bkind != BK_NONE) {
+            a.B = P;
+            a.Bn = Pn;
+        }
         hipError_t e = p->precision == SB_PREC_ORDERED ? launch_step_ordered(a, s)
                                                        : launch_step(family, shape, a, s);
         if (e != hipSuccess) return hip_fail(e, "launch_step");
         std::swap(X, Xn);
+        std::swap(P, Pn);
     }
     HIP_TRY(hipEventRecord(ctx->ev1, s));
     rc = read_results(ctx, prob, X, Y, R, res, s);

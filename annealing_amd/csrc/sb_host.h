@@ -17,6 +17,9 @@ struct sb_ctx {
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;
     size_t state_cap = 0;  // floats per buffer
+    // B-operand planes of X0 / X1 (4 bytes per state element: BF16 hi + lo).
+    unsigned char *P0 = nullptr, *P1 = nullptr;
+    size_t plane_cap = 0;  // bytes per buffer
     double *H = nullptr, *part = nullptr;
     size_t H_cap = 0, part_cap = 0;
     int8_t *S = nullptr;
@@ -26,6 +29,7 @@ struct sb_ctx {
     // Autotuned (family, shape) per (family set, N bucket, R bucket).
     std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tune;
     bool autotune = true;
+    bool verbose = false;   // SB_VERBOSE=1: log autotune timings to stderr
     int live_problems = 0;  // guarded by mu
 };
 

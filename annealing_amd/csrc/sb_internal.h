@@ -17,16 +17,43 @@ constexpr int kSpinPad = 64;
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // Arguments of one fused step over one instance.
+// B-operand planes: the field kernels' B operand prepared once per step by
+// the previous step's epilogue (or k_make_planes after init), so staging it
+// is a plain copy.  Layout [R][ldx] per plane.
+enum BKind {
+    BK_NONE = 0,      // B staged from X (FP32 family, or split without planes)
+    BK_HILO = 1,      // BF16 hi plane, then BF16 lo plane (bplane elements later)
+    BK_SIGN_FP8 = 2,  // E4M3 +-1 bytes
+    BK_SIGN_BF16 = 3  // BF16 +-1
+};
+
 struct StepArgs {
     const void *J;   // [N][ldj] in the kernel's J format
     const float *X;  // [R][ldx] state read this step
     float *Xn;       // [R][ldx] state written this step
     float *Y;        // [R][ldx] updated in place
+    const void *B;   // B-operand planes of X (bkind != BK_NONE)
+    void *Bn;        // planes of Xn written by the epilogue (may be null)
+    size_t bplane;   // elements per plane (R * ldx)
+    int bkind;
     int ldj, ldx;
     int N, R;
     int tiles_m, tiles_n;  // row tiles x replica tiles of this launch
     sb_step_coeffs c;
 };
+
+// Plane kind a family reads for a variant (BK_NONE: stages from X).
+inline int plane_kind(int family, int variant)
+{
+    if (family == 1 || family == 2) return variant == SB_DSB ? BK_SIGN_BF16 : BK_HILO;
+    if (family == 3) return BK_SIGN_FP8;
+    return BK_NONE;
+}
+// Bytes per state element of a plane set of this kind.
+inline size_t plane_bytes(int kind)
+{
+    return kind == BK_HILO ? 4 : kind == BK_SIGN_BF16 ? 2 : kind == BK_SIGN_FP8 ? 1 : 0;
+}
 
 // Tile shapes.  A workgroup has WM x WN waves; each wave owns BM x BN blocks
 // of 32 x 32 (spins x replicas).  Every shape accumulates each F_i in the same
@@ -54,6 +81,8 @@ TileShape shape_at(int family, int idx);
 // Launchers (sb_kernels.hip).  Return a hipError_t.
 hipError_t launch_step(int family, int shape_idx, const StepArgs &a, hipStream_t s);
 hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s);
+// Planes of kind `kind` for state X ([R][ldx]) into B.
+hipError_t launch_make_planes(int kind, const float *X, int R, int ldx, void *B, hipStream_t s);
 hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X, float *Y,
                        float *X2, hipStream_t s);
 hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, float *X, float *Y,

@@ -81,6 +81,45 @@ __device__ __forceinline__ void sb_update(const sb_step_coeffs &c, float F, floa
 
 __device__ __forceinline__ float sgn1(float v) { return v > 0.0f ? 1.0f : -1.0f; }
 
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
+}
+__device__ __forceinline__ float bf2f(unsigned short h)
+{
+    return __builtin_bit_cast(float, (unsigned int)h << 16);
+}
+
+// B-operand planes of four consecutive spins (one float4 of X) at element o:
+//   BK_HILO      hi = bf16(x), lo = bf16(x - hi): x = hi + lo to 2^-17 |x|
+//   BK_SIGN_FP8  E4M3 +1.0 = 0x38, -1.0 = 0xB8 (x > 0 ? +1 : -1)
+//   BK_SIGN_BF16 BF16 +1.0 = 0x3F80, -1.0 = 0xBF80
+__device__ __forceinline__ void write_planes(int kind, const v4f &xv, void *B, size_t bplane, size_t o)
+{
+    if (kind == BK_HILO) {
+        u16x4 h, l;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            unsigned short hb = f2bf(xv[e]);
+            h[e] = hb;
+            l[e] = f2bf(xv[e] - bf2f(hb));
+        }
+        unsigned short *p = (unsigned short *)B;
+        *(u16x4 *)(p + o) = h;
+        *(u16x4 *)(p + bplane + o) = l;
+    } else if (kind == BK_SIGN_FP8) {
+        unsigned int s4 = 0u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s4 |= (xv[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
+        *(unsigned int *)((unsigned char *)B + o) = s4;
+    } else if (kind == BK_SIGN_BF16) {
+        u16x4 h;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[e] = xv[e] > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
+        *(u16x4 *)((unsigned short *)B + o) = h;
+    }
+}
+
 // Epilogue of one 32x32 accumulator block.  MFMA 32x32 output layout: lane l,
 // register v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32, so each lane
 // owns four runs of four consecutive spins of one replica: one float4 each.
@@ -107,6 +146,7 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
         }
         *(v4f *)(xnr + i) = xv;
         *(v4f *)(yr + i) = yv;
+        if (a.Bn) write_planes(a.bkind, xv, a.Bn, a.bplane, (size_t)r * a.ldx + i);
     }
 }
 
@@ -230,30 +270,19 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
 
 // ---------------------------------------------------------------------------
 // Split family: v_mfma_f32_32x32x16_bf16.  K chunk of 64.  A = J in BF16 (read
-// as BF16, or as FP8 E4M3 widened to BF16 while staging).  B = X split into
-// BF16 hi = bf16(x) and lo = bf16(x - hi) while staging, two MFMA passes into
-// one FP32 accumulator per k16 group (hi then lo); dSB stages sign(x) once.
-// Lane half h reads 8 consecutive k (16 bytes) at 16kk + 8h for A and B.
+// as BF16, or as FP8 E4M3 widened to BF16 by v_cvt_scalef32_pk_bf16_fp8 while
+// staging).  B = X as BF16 hi = bf16(x) and lo = bf16(x - hi), two MFMA passes
+// into one FP32 accumulator per k16 group (hi then lo); dSB uses sign(x) once.
+// With PLANES the B operand comes ready-made from planes written by the
+// previous step's epilogue; without, it is split from X while staging (same
+// values).  Lane half h reads 8 consecutive k (16 bytes) at 16kk + 8h for A
+// and B.
 
-__device__ __forceinline__ unsigned short f2bf(float f)
+// 4 FP8 bytes -> 4 BF16 (exact for the integers |J| <= 16 the packer stores).
+__device__ __forceinline__ void fp8x4_to_bf16x4(unsigned int b4, unsigned int &lo2, unsigned int &hi2)
 {
-    return __builtin_bit_cast(unsigned short, (__bf16)f);
-}
-__device__ __forceinline__ float bf2f(unsigned short h)
-{
-    return __builtin_bit_cast(float, (unsigned int)h << 16);
-}
-
-// FP8 E4M3 byte -> BF16 bits for normal numbers and zero (the packer stores
-// only integers |J| <= 16, which are all normal): exponent bias 7 -> 127 is an
-// add of 120 to the exponent field, mantissa 3 -> 7 bits a shift.
-__device__ __forceinline__ unsigned int fp8pair_to_bf16x2(unsigned int b2)
-{
-    unsigned int lo = b2 & 0xffu, hi = (b2 >> 8) & 0xffu;
-    unsigned int ml = lo & 0x7fu, mh = hi & 0x7fu;
-    unsigned int rl = ((lo & 0x80u) << 8) | (ml ? (ml << 4) + (120u << 7) : 0u);
-    unsigned int rh = ((hi & 0x80u) << 8) | (mh ? (mh << 4) + (120u << 7) : 0u);
-    return rl | (rh << 16);
+    lo2 = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(b4, 1.0f, false));
+    hi2 = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(b4, 1.0f, true));
 }
 
 template <int WM, int WN, int BM, int BN>
@@ -262,7 +291,7 @@ struct SplitTile {
     static constexpr int KC = 64, LDS_LD = KC + 8;  // in bf16 elements
 };
 
-template <int WM, int WN, int BM, int BN, bool FP8J>
+template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES>
 __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned short *smem)
 {
     using T = SplitTile<WM, WN, BM, BN>;
@@ -271,7 +300,9 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     constexpr int A_PER = FP8J ? 16 : 8;               // elements per global load
     constexpr int A_LOADS = MT * KC / A_PER;
     constexpr int LA = (A_LOADS + NTHR - 1) / NTHR;
-    constexpr int B_LOADS = NT * KC / 4;                // float4 loads of X
+    // B: float4 loads of X, or 16-byte loads of 8 BF16 per plane.
+    constexpr int B_PER = PLANES ? 8 : 4;
+    constexpr int B_LOADS = NT * KC / B_PER;
     constexpr int LB = (B_LOADS + NTHR - 1) / NTHR;
     unsigned short *As = smem;                          // [2][MT][LD]
     unsigned short *Bh = smem + 2 * MT * LD;            // [2][NT][LD]
@@ -282,9 +313,12 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     const int wm = wave % WM, wn = wave / WM;
     const bool sgn = a.c.variant == SB_DSB;
     const int Kp = round_up_dev(a.N, KC);
+    const unsigned short *Ph = (const unsigned short *)a.B;
+    const unsigned short *Pl = Ph + a.bplane;
 
     u32x4 ra[LA];
-    v4f rb[LB];
+    v4f rb[PLANES ? 1 : LB];
+    u32x4 rbh[PLANES ? LB : 1], rbl[PLANES ? LB : 1];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
@@ -303,11 +337,22 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
-            int rr = q / (KC / 4), c4 = q % (KC / 4);
+            int rr = q / (KC / B_PER), c = q % (KC / B_PER);
             int gr = col0 + rr;
-            v4f v = {0.f, 0.f, 0.f, 0.f};
-            if (q < B_LOADS && gr < a.R) v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c4);
-            rb[u] = v;
+            if (PLANES) {
+                u32x4 h = {0u, 0u, 0u, 0u}, l = {0u, 0u, 0u, 0u};
+                if (q < B_LOADS && gr < a.R) {
+                    size_t o = (size_t)gr * a.ldx + k0 + 8 * c;
+                    h = *(const u32x4 *)(Ph + o);
+                    if (!sgn) l = *(const u32x4 *)(Pl + o);
+                }
+                rbh[u] = h;
+                rbl[u] = l;
+            } else {
+                v4f v = {0.f, 0.f, 0.f, 0.f};
+                if (q < B_LOADS && gr < a.R) v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c);
+                rb[u] = v;
+            }
         }
     };
     auto sstore = [&](int buf) {
@@ -318,17 +363,11 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
                 int rr = q / (KC / A_PER), c = q % (KC / A_PER);
                 unsigned short *dst = As + (buf * MT + rr) * LD + A_PER * c;
                 if (FP8J) {
-                    u32x4 lo4, hi4;
-                    lo4[0] = fp8pair_to_bf16x2(ra[u][0]);
-                    lo4[1] = fp8pair_to_bf16x2(ra[u][0] >> 16);
-                    lo4[2] = fp8pair_to_bf16x2(ra[u][1]);
-                    lo4[3] = fp8pair_to_bf16x2(ra[u][1] >> 16);
-                    hi4[0] = fp8pair_to_bf16x2(ra[u][2]);
-                    hi4[1] = fp8pair_to_bf16x2(ra[u][2] >> 16);
-                    hi4[2] = fp8pair_to_bf16x2(ra[u][3]);
-                    hi4[3] = fp8pair_to_bf16x2(ra[u][3] >> 16);
-                    *(u32x4 *)dst = lo4;
-                    *(u32x4 *)(dst + 8) = hi4;
+                    unsigned int w[8];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) fp8x4_to_bf16x4(ra[u][e], w[2 * e], w[2 * e + 1]);
+                    *(u32x4 *)dst = (u32x4){w[0], w[1], w[2], w[3]};
+                    *(u32x4 *)(dst + 8) = (u32x4){w[4], w[5], w[6], w[7]};
                 } else {
                     *(u32x4 *)dst = ra[u];
                 }
@@ -338,22 +377,27 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
             if (q < B_LOADS) {
-                int rr = q / (KC / 4), c4 = q % (KC / 4);
-                u16x4 h, l;
+                int rr = q / (KC / B_PER), c = q % (KC / B_PER);
+                if (PLANES) {
+                    *(u32x4 *)(Bh + (buf * NT + rr) * LD + 8 * c) = rbh[u];
+                    if (!sgn) *(u32x4 *)(Bl + (buf * NT + rr) * LD + 8 * c) = rbl[u];
+                } else {
+                    u16x4 h, l;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float v = rb[u][e];
-                    if (sgn) {
-                        h[e] = v > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
-                        l[e] = 0;
-                    } else {
-                        unsigned short hb = f2bf(v);
-                        h[e] = hb;
-                        l[e] = f2bf(v - bf2f(hb));
+                    for (int e = 0; e < 4; ++e) {
+                        float v = rb[u][e];
+                        if (sgn) {
+                            h[e] = v > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
+                            l[e] = 0;
+                        } else {
+                            unsigned short hb = f2bf(v);
+                            h[e] = hb;
+                            l[e] = f2bf(v - bf2f(hb));
+                        }
                     }
+                    *(u16x4 *)(Bh + (buf * NT + rr) * LD + 4 * c) = h;
+                    if (!sgn) *(u16x4 *)(Bl + (buf * NT + rr) * LD + 4 * c) = l;
                 }
-                *(u16x4 *)(Bh + (buf * NT + rr) * LD + 4 * c4) = h;
-                if (!sgn) *(u16x4 *)(Bl + (buf * NT + rr) * LD + 4 * c4) = l;
             }
         }
     };
@@ -425,14 +469,16 @@ struct Fp8Tile {
     static constexpr int KC = 128, LDS_LD = KC + 16;  // bytes
 };
 
-template <int WM, int WN, int BM, int BN>
+template <int WM, int WN, int BM, int BN, bool PLANES>
 __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsigned char *smem)
 {
     using T = Fp8Tile<WM, WN, BM, BN>;
     constexpr int MT = T::MT, NT = T::NT, NTHR = T::NTHR, KC = T::KC, LD = T::LDS_LD;
     constexpr int A_LOADS = MT * KC / 16;
     constexpr int LA = (A_LOADS + NTHR - 1) / NTHR;
-    constexpr int B_LOADS = NT * KC / 4;  // float4 loads of X -> 4 sign bytes
+    // B: float4 loads of X -> 4 sign bytes, or 16-byte loads of the sign plane.
+    constexpr int B_PER = PLANES ? 16 : 4;
+    constexpr int B_LOADS = NT * KC / B_PER;
     constexpr int LB = (B_LOADS + NTHR - 1) / NTHR;
     unsigned char *As = smem;               // [2][MT][LD]
     unsigned char *Bs = smem + 2 * MT * LD; // [2][NT][LD]
@@ -440,12 +486,14 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
     const int row0 = tm * MT, col0 = tn * NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    // K loop bound: multiple of KC inside the ld padding (ld is a multiple of 64;
-    // reads past ld are avoided by clamping the last chunk's length).
+    // K loop bound: the last 128-byte chunk is clipped at round_up(N, 64) <= ld
+    // (zero bytes beyond it; J is zero there).
     const int Kp = round_up_dev(a.N, 64);
+    const unsigned char *P = (const unsigned char *)a.B;
 
     u32x4 ra[LA];
-    unsigned int rb[LB];
+    unsigned int rb[PLANES ? 1 : LB];
+    u32x4 rbp[PLANES ? LB : 1];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
@@ -460,16 +508,22 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
-            int rr = q / (KC / 4), c4 = q % (KC / 4);
-            int gr = col0 + rr, gk = k0 + 4 * c4;
-            unsigned int s4 = 0u;  // FP8 zero bytes beyond N: J is zero there anyway
-            if (q < B_LOADS && gr < a.R && gk < Kp) {
-                v4f v = *(const v4f *)(a.X + (size_t)gr * a.ldx + gk);
-                // E4M3 +1.0 = 0x38, -1.0 = 0xB8
+            int rr = q / (KC / B_PER), c = q % (KC / B_PER);
+            int gr = col0 + rr, gk = k0 + B_PER * c;
+            if (PLANES) {
+                u32x4 v = {0u, 0u, 0u, 0u};
+                if (q < B_LOADS && gr < a.R && gk < Kp) v = *(const u32x4 *)(P + (size_t)gr * a.ldx + gk);
+                rbp[u] = v;
+            } else {
+                unsigned int s4 = 0u;  // FP8 zero bytes beyond N: J is zero there anyway
+                if (q < B_LOADS && gr < a.R && gk < Kp) {
+                    v4f v = *(const v4f *)(a.X + (size_t)gr * a.ldx + gk);
+                    // E4M3 +1.0 = 0x38, -1.0 = 0xB8
 #pragma unroll
-                for (int e = 0; e < 4; ++e) s4 |= (v[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
+                    for (int e = 0; e < 4; ++e) s4 |= (v[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
+                }
+                rb[u] = s4;
             }
-            rb[u] = s4;
         }
     };
     auto sstore = [&](int buf) {
@@ -485,8 +539,11 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
             if (q < B_LOADS) {
-                int rr = q / (KC / 4), c4 = q % (KC / 4);
-                *(unsigned int *)(Bs + (buf * NT + rr) * LD + 4 * c4) = rb[u];
+                int rr = q / (KC / B_PER), c = q % (KC / B_PER);
+                if (PLANES)
+                    *(u32x4 *)(Bs + (buf * NT + rr) * LD + 16 * c) = rbp[u];
+                else
+                    *(unsigned int *)(Bs + (buf * NT + rr) * LD + 4 * c) = rb[u];
             }
         }
     };
@@ -556,20 +613,20 @@ __global__ __launch_bounds__(64 * WM * WN) void k_step_fp32(StepArgs a)
     fp32_tile<WM, WN, BM, BN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
-template <int WM, int WN, int BM, int BN, bool FP8J>
+template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES>
 __global__ __launch_bounds__(64 * WM * WN) void k_step_split(StepArgs a)
 {
     using T = SplitTile<WM, WN, BM, BN>;
     __shared__ __attribute__((aligned(16))) unsigned short smem[2 * (T::MT + 2 * T::NT) * T::LDS_LD];
-    split_tile<WM, WN, BM, BN, FP8J>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+    split_tile<WM, WN, BM, BN, FP8J, PLANES>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
-template <int WM, int WN, int BM, int BN>
+template <int WM, int WN, int BM, int BN, bool PLANES>
 __global__ __launch_bounds__(64 * WM * WN) void k_step_dsb_fp8(StepArgs a)
 {
     using T = Fp8Tile<WM, WN, BM, BN>;
     __shared__ __attribute__((aligned(16))) unsigned char smem[2 * (T::MT + T::NT) * T::LDS_LD];
-    dsb_fp8_tile<WM, WN, BM, BN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+    dsb_fp8_tile<WM, WN, BM, BN, PLANES>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
 // Ordered-sum debug kernel: one thread per (spin, replica), F_i summed over
@@ -595,11 +652,14 @@ __global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Shape tables.  All shapes use 4 waves (256 threads).
+// Shape tables (4-wave and 8-wave workgroups).
 
-#define SB_FP32_SHAPES(X) X(2, 2, 1, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1)
-#define SB_SPLIT_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 1, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1)
-#define SB_FP8_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 1, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1)
+#define SB_FP32_SHAPES(X) X(2, 2, 1, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1) \
+    X(2, 4, 1, 1) X(4, 2, 1, 1) X(4, 2, 1, 2) X(2, 4, 2, 1) X(4, 2, 2, 1)
+#define SB_SPLIT_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
+    X(4, 2, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1)
+#define SB_FP8_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
+    X(4, 2, 2, 2) X(2, 4, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1)
 
 #define SB_SHAPE_ENTRY(wm, wn, bm, bn) TileShape{wm, wn, bm, bn},
 static const TileShape kFp32Shapes[] = {SB_FP32_SHAPES(SB_SHAPE_ENTRY)};
@@ -631,13 +691,19 @@ TileShape shape_at(int family, int idx)
 typedef void (*StepKernel)(StepArgs);
 
 #define SB_FP32_PTR(wm, wn, bm, bn) &k_step_fp32<wm, wn, bm, bn>,
-#define SB_SPLIT_BF16_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false>,
-#define SB_SPLIT_FP8_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true>,
-#define SB_DSB_FP8_PTR(wm, wn, bm, bn) &k_step_dsb_fp8<wm, wn, bm, bn>,
+#define SB_SPLIT_BF16_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, false>,
+#define SB_SPLIT_FP8_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, false>,
+#define SB_DSB_FP8_PTR(wm, wn, bm, bn) &k_step_dsb_fp8<wm, wn, bm, bn, false>,
+#define SB_SPLIT_BF16_PL_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, true>,
+#define SB_SPLIT_FP8_PL_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true>,
+#define SB_DSB_FP8_PL_PTR(wm, wn, bm, bn) &k_step_dsb_fp8<wm, wn, bm, bn, true>,
 static const StepKernel kFp32Kernels[] = {SB_FP32_SHAPES(SB_FP32_PTR)};
 static const StepKernel kSplitBf16Kernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_BF16_PTR)};
 static const StepKernel kSplitFp8Kernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_FP8_PTR)};
 static const StepKernel kDsbFp8Kernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PTR)};
+static const StepKernel kSplitBf16PlKernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_BF16_PL_PTR)};
+static const StepKernel kSplitFp8PlKernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_FP8_PL_PTR)};
+static const StepKernel kDsbFp8PlKernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PL_PTR)};
 
 hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStream_t s)
 {
@@ -648,12 +714,15 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
     a.tiles_n = (a.R + sh.nt() - 1) / sh.nt();
     int grid = a.tiles_m * a.tiles_n;
     if (grid <= 0) return hipSuccess;
+    // Planes are read only when the caller supplies ones of the family's kind.
+    const bool pl = a.B != nullptr && a.bkind == plane_kind(family, a.c.variant) && a.bkind != BK_NONE;
+    if (a.Bn && a.bkind != plane_kind(family, a.c.variant)) return hipErrorInvalidValue;
     StepKernel k = nullptr;
     switch (family) {
     case FAM_FP32: k = kFp32Kernels[shape_idx]; break;
-    case FAM_SPLIT_BF16: k = kSplitBf16Kernels[shape_idx]; break;
-    case FAM_SPLIT_FP8: k = kSplitFp8Kernels[shape_idx]; break;
-    case FAM_DSB_FP8: k = kDsbFp8Kernels[shape_idx]; break;
+    case FAM_SPLIT_BF16: k = pl ? kSplitBf16PlKernels[shape_idx] : kSplitBf16Kernels[shape_idx]; break;
+    case FAM_SPLIT_FP8: k = pl ? kSplitFp8PlKernels[shape_idx] : kSplitFp8Kernels[shape_idx]; break;
+    case FAM_DSB_FP8: k = pl ? kDsbFp8PlKernels[shape_idx] : kDsbFp8Kernels[shape_idx]; break;
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k, dim3(grid), dim3(sh.threads()), 0, s, a);
@@ -665,6 +734,23 @@ hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
     if (a.N <= 0 || a.R <= 0) return hipSuccess;
     dim3 grid((a.N + 255) / 256, a.R);
     hipLaunchKernelGGL(k_step_ordered, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_make_planes(int kind, const float *X, size_t n4, void *B, size_t bplane)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (size_t)gridDim.x * blockDim.x) {
+        v4f xv = *(const v4f *)(X + 4 * q);
+        write_planes(kind, xv, B, bplane, 4 * q);
+    }
+}
+
+hipError_t launch_make_planes(int kind, const float *X, int R, int ldx, void *B, hipStream_t s)
+{
+    if (kind == BK_NONE || R <= 0) return hipSuccess;
+    size_t n = (size_t)R * ldx, n4 = n / 4;
+    size_t g = (n4 + 255) / 256;
+    hipLaunchKernelGGL(k_make_planes, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(256), 0, s, kind, X, n4, B, n);
     return hipGetLastError();
 }
 

@@ -39,8 +39,15 @@ struct StepArgs {
     int ldj, ldx;
     int N, R;
     int tiles_m, tiles_n;  // row tiles x replica tiles of this launch
+    int rpg;               // row tiles per XCD group of the tile order (0: all)
     sb_step_coeffs c;
 };
+
+// Row tiles per group of the XCD-blocked tile order: each of the 8 XCDs runs
+// ~T/8 consecutive logical tiles; a block of g rows x (T/8)/g columns needs
+// g*MT*bytesJ + (T/8/g)*NT*bytesB bytes per K, minimal at
+// g = sqrt(T/8 * NT*bytesB / (MT*bytesJ)).
+int rows_per_group(int tiles_m, int tiles_n, int mt, int nt, int bytes_j, int bytes_b);
 
 // Plane kind a family reads for a variant (BK_NONE: stages from X).
 inline int plane_kind(int family, int variant)

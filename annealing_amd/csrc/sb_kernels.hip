@@ -27,9 +27,11 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// Workgroup -> tile mapping.  Workgroup b is dispatched to XCD b % 8; give
-// each XCD a contiguous range of logical tiles so tiles that share replica
-// columns (tm fastest) run on one XCD and share its L2.
+// Workgroup -> tile mapping.  Workgroup b is dispatched to XCD b % 8; xcd_remap
+// gives each XCD a contiguous range of logical tiles, and tile_coords lays the
+// logical tiles out in groups of `rpg` row tiles (row fastest inside a group),
+// so one XCD's range is a near-square block of rows x replica columns whose J
+// rows and X columns stay in that XCD's L2.
 __device__ __forceinline__ int round_up_dev(int v, int m) { return (v + m - 1) / m * m; }
 
 __device__ __forceinline__ int xcd_remap(int bid, int nblk)
@@ -37,6 +39,17 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk)
     int xcd = bid & 7, local = bid >> 3;
     int per = nblk >> 3, rem = nblk & 7;
     return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+__device__ __forceinline__ void tile_coords(const StepArgs &a, int L, int &tm, int &tn)
+{
+    const int rpg = a.rpg > 0 ? a.rpg : a.tiles_m;
+    const int full = rpg * a.tiles_n;
+    const int g = L / full;
+    const int rg = min(rpg, a.tiles_m - g * rpg);
+    const int local = L - g * full;
+    tn = local / rg;
+    tm = g * rpg + local % rg;
 }
 
 // ---------------------------------------------------------------------------
@@ -172,7 +185,8 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
     constexpr int MT = T::MT, NT = T::NT, NTHR = T::NTHR, KC = T::KC, LD = T::LDS_LD;
     float *As = smem;                  // [2][MT][LD]
     float *Bs = smem + 2 * MT * LD;    // [2][NT][LD]
-    const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
+    int tm, tn;
+    tile_coords(a, tile, tm, tn);
     const int row0 = tm * MT, col0 = tn * NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
@@ -307,7 +321,8 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     unsigned short *As = smem;                          // [2][MT][LD]
     unsigned short *Bh = smem + 2 * MT * LD;            // [2][NT][LD]
     unsigned short *Bl = Bh + 2 * NT * LD;              // [2][NT][LD]
-    const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
+    int tm, tn;
+    tile_coords(a, tile, tm, tn);
     const int row0 = tm * MT, col0 = tn * NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
@@ -482,7 +497,8 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
     constexpr int LB = (B_LOADS + NTHR - 1) / NTHR;
     unsigned char *As = smem;               // [2][MT][LD]
     unsigned char *Bs = smem + 2 * MT * LD; // [2][NT][LD]
-    const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
+    int tm, tn;
+    tile_coords(a, tile, tm, tn);
     const int row0 = tm * MT, col0 = tn * NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
@@ -688,6 +704,16 @@ TileShape shape_at(int family, int idx)
     }
 }
 
+int rows_per_group(int tiles_m, int tiles_n, int mt, int nt, int bytes_j, int bytes_b)
+{
+    double per_xcd = (double)tiles_m * tiles_n / 8.0;
+    double g = std::sqrt(per_xcd * nt * bytes_b / ((double)mt * bytes_j));
+    int r = (int)(g + 0.5);
+    if (r < 1) r = 1;
+    if (r > tiles_m) r = tiles_m;
+    return r;
+}
+
 typedef void (*StepKernel)(StepArgs);
 
 #define SB_FP32_PTR(wm, wn, bm, bn) &k_step_fp32<wm, wn, bm, bn>,
@@ -717,6 +743,9 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
     // Planes are read only when the caller supplies ones of the family's kind.
     const bool pl = a.B != nullptr && a.bkind == plane_kind(family, a.c.variant) && a.bkind != BK_NONE;
     if (a.Bn && a.bkind != plane_kind(family, a.c.variant)) return hipErrorInvalidValue;
+    const int bytes_j = family == FAM_FP32 ? 4 : family == FAM_SPLIT_BF16 ? 2 : 1;
+    const int bytes_b = pl ? (int)plane_bytes(a.bkind) : 4;
+    a.rpg = rows_per_group(a.tiles_m, a.tiles_n, sh.mt(), sh.nt(), bytes_j, bytes_b);
     StepKernel k = nullptr;
     switch (family) {
     case FAM_FP32: k = kFp32Kernels[shape_idx]; break;

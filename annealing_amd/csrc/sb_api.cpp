@@ -284,24 +284,7 @@ extern "C" SB_API int sb_generate(int32_t kind, int32_t N, double p, uint64_t se
 extern "C" SB_API int sb_step_coeffs_for(const sb_params *p, double c0, int32_t k, sb_step_coeffs *out)
 {
     if (!p || !out || p->steps < 1 || k < 0 || k >= p->steps) return SB_E_INVALID;
-    std::memset(out, 0, sizeof(*out));
-    out->variant = p->variant;
-    out->M = p->M;
-    out->a0 = p->a0;
-    out->K = p->K;
-    out->dts = p->M > 0 ? p->dt / (double)p->M : 0.0;
-    if (p->variant == SB_ASB) {
-        out->beta = (float)(p->dt * c0);  // m2_opt.cu:148 coeff1 = DeltaT * xi
-        out->do_local = (k + 1 < p->steps) ? 1 : 0;
-        out->p_next = (k + 1 < p->steps) ? (float)(k + 1) / (float)p->steps : 0.0f;  // m2_opt.cu:158
-    } else {
-        double pk = (double)k / (double)p->steps;
-        double a = p->a0 * pk;
-        out->alpha = (float)(p->dt * (p->a0 - a));
-        out->beta = (float)(p->dt * c0);
-        out->gamma = (float)(p->dt * p->a0);
-        out->p_next = (float)pk;
-    }
+    *out = sbk::step_coeffs(p->variant, k, p->steps, p->dt, p->a0, c0, p->K, p->M);
     return SB_OK;
 }
 
@@ -607,28 +590,14 @@ static int heuristic_shape(int family, int N, int R)
     return best;
 }
 
-int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int variant, int R, int *family,
-                  int *shape)
+// Time (family, shape) candidates for a few launches each on the context
+// workspace (its contents are overwritten by the init that follows) and return
+// the fastest.  The state is random, not zero: MFMA timing depends on the
+// operand values (power).
+static int tune_candidates(sb_ctx *ctx, const sb_problem *prob, int variant, int R,
+                           const std::vector<std::pair<int, int>> &cands, std::pair<int, int> *best)
 {
-    std::vector<int> fams = candidate_families(precision, variant, prob);
-    if (fams.empty()) return SB_E_INVALID;
     const int N = prob->N;
-    auto key = std::make_tuple(fams[0] * 8 + precision, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
-    auto it = ctx->tune.find(key);
-    if (it != ctx->tune.end()) {
-        *family = it->second.first;
-        *shape = it->second.second;
-        return SB_OK;
-    }
-    if (!ctx->autotune) {
-        *family = fams[0];
-        *shape = heuristic_shape(fams[0], N, R);
-        ctx->tune[key] = {*family, *shape};
-        return SB_OK;
-    }
-    // Time each candidate for a few launches on the workspace (its contents
-    // are overwritten by the init that follows).  The state is random, not
-    // zero: MFMA timing depends on the operand values (power).
     int rc = ensure_workspace(ctx, (size_t)R * prob->ld, R, N);
     if (rc) return rc;
     HIP_TRY(launch_init(SB_INIT_UNIFORM, 0x5eed, N, R, prob->ld, ctx->X0, ctx->Y, ctx->X1, ctx->stream));
@@ -642,39 +611,114 @@ int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int varian
     a.R = R;
     a.c.variant = variant;
     a.c.M = 1;
-    float best_ms = 1e30f;
-    int best_f = fams[0], best_s = 0;
-    const int reps = 3;
     a.bplane = (size_t)R * prob->ld;
-    for (int f : fams) {
+    float best_ms = 1e30f;
+    *best = cands.front();
+    const int reps = 3;
+    int planes_of = -1;
+    for (const auto &fs : cands) {
+        const int f = fs.first, s = fs.second;
         a.J = family_J(f, prob);
         a.bkind = plane_kind(f, variant);
         a.B = a.bkind != BK_NONE ? (const void *)ctx->P0 : nullptr;
         a.Bn = a.bkind != BK_NONE ? (void *)ctx->P1 : nullptr;
-        if (a.bkind != BK_NONE) HIP_TRY(launch_make_planes(a.bkind, ctx->X0, R, prob->ld, ctx->P0, ctx->stream));
-        for (int s = 0; s < shape_count(f); ++s) {
-            HIP_TRY(launch_step(f, s, a, ctx->stream));  // warm-up
-            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-            for (int r = 0; r < reps; ++r) HIP_TRY(launch_step(f, s, a, ctx->stream));
-            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-            HIP_TRY(hipEventSynchronize(ctx->ev1));
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            if (ctx->verbose) {
-                TileShape sh = shape_at(f, s);
-                std::fprintf(stderr, "[sb autotune] N=%d R=%d family=%d shape=%d (%dx%d, %d thr) %.2f us\n", N, R, f,
-                             s, sh.mt(), sh.nt(), sh.threads(), ms * 1e3f / reps);
-            }
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_f = f;
-                best_s = s;
-            }
+        if (a.bkind != BK_NONE && planes_of != a.bkind) {
+            HIP_TRY(launch_make_planes(a.bkind, ctx->X0, R, prob->ld, ctx->P0, ctx->stream));
+            HIP_TRY(launch_make_planes(a.bkind, ctx->X0, R, prob->ld, ctx->P1, ctx->stream));
+            planes_of = a.bkind;
+        }
+        HIP_TRY(launch_step(f, s, a, ctx->stream));  // warm-up
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        for (int r = 0; r < reps; ++r) HIP_TRY(launch_step(f, s, a, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ctx->verbose) {
+            TileShape sh = shape_at(f, s);
+            std::fprintf(stderr, "[sb autotune] N=%d R=%d family=%d shape=%d (%dx%d, %d thr) %.2f us\n", N, R, f, s,
+                         sh.mt(), sh.nt(), sh.threads(), ms * 1e3f / reps);
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = fs;
         }
     }
-    ctx->tune[key] = {best_f, best_s};
-    *family = best_f;
-    *shape = best_s;
+    return SB_OK;
+}
+
+int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int variant, int R, int *family,
+                  int *shape)
+{
+    std::vector<int> fams = candidate_families(precision, variant, prob);
+    if (fams.empty()) return SB_E_INVALID;
+    const int N = prob->N;
+    // Debug override SB_SHAPE=<family>:<shape> (used by the shape-sweep tests).
+    if (const char *force = std::getenv("SB_SHAPE")) {
+        int f = -1, s = -1;
+        if (std::sscanf(force, "%d:%d", &f, &s) == 2 && std::find(fams.begin(), fams.end(), f) != fams.end() &&
+            s >= 0 && s < shape_count(f)) {
+            *family = f;
+            *shape = s;
+            return SB_OK;
+        }
+    }
+    auto key = std::make_tuple(fams[0] * 8 + precision, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
+    auto it = ctx->tune.find(key);
+    if (it != ctx->tune.end()) {
+        *family = it->second.first;
+        *shape = it->second.second;
+        return SB_OK;
+    }
+    std::pair<int, int> best{fams[0], heuristic_shape(fams[0], N, R)};
+    if (ctx->autotune) {
+        std::vector<std::pair<int, int>> cands;
+        for (int f : fams)
+            for (int s = 0; s < shape_count(f); ++s) cands.push_back({f, s});
+        int rc = tune_candidates(ctx, prob, variant, R, cands, &best);
+        if (rc) return rc;
+    }
+    ctx->tune[key] = best;
+    *family = best.first;
+    *shape = best.second;
+    return SB_OK;
+}
+
+int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int variant, int R, int *bshape)
+{
+    const int N = prob->N;
+    // Cache key: family + 100 marks the batched-shape table.
+    auto key = std::make_tuple(family + 100, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
+    auto it = ctx->tune.find(key);
+    if (it != ctx->tune.end()) {
+        *bshape = it->second.second;
+        return SB_OK;
+    }
+    const int nb = batch_shape_count(family);
+    if (nb <= 0) return SB_E_INVALID;
+    // Debug override SB_BATCH_SHAPE=<index> (used by the shape-sweep tests).
+    if (const char *force = std::getenv("SB_BATCH_SHAPE")) {
+        int b = std::atoi(force);
+        if (b >= 0 && b < nb) {
+            *bshape = b;
+            return SB_OK;
+        }
+    }
+    int pick = 0;
+    if (ctx->autotune) {
+        std::vector<std::pair<int, int>> cands;
+        for (int b = 0; b < nb; ++b) cands.push_back({family, batch_shape_solo_index(family, b)});
+        std::pair<int, int> best;
+        int rc = tune_candidates(ctx, prob, variant, R, cands, &best);
+        if (rc) return rc;
+        for (int b = 0; b < nb; ++b)
+            if (batch_shape_solo_index(family, b) == best.second) pick = b;
+    } else {
+        // smallest tile with at least a few tiles per instance
+        pick = 0;
+    }
+    ctx->tune[key] = {family, pick};
+    *bshape = pick;
     return SB_OK;
 }
 
@@ -746,7 +790,13 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     a.bkind = p->precision == SB_PREC_ORDERED ? BK_NONE : plane_kind(family, p->variant);
     a.bplane = (size_t)R * ld;
     unsigned char *P = ctx->P0, *Pn = ctx->P1;
-    if (a.bkind != BK_NONE) HIP_TRY(launch_make_planes(a.bkind, X, R, ld, P, s));
+    // Both plane buffers start complete: the epilogue rewrites only the
+    // 4-spin groups holding real spins, and the K loop reads the padding
+    // columns too (J is zero there, but 0 * NaN from stale bytes is NaN).
+    if (a.bkind != BK_NONE) {
+        HIP_TRY(launch_make_planes(a.bkind, X, R, ld, P, s));
+        HIP_TRY(launch_make_planes(a.bkind, X, R, ld, Pn, s));
+    }
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     for (int k = 0; k < p->steps; ++k) {
         sb_step_coeffs_for(p, c0, k, &a.c);

@@ -1,0 +1,373 @@
+// sb_engine.cpp -- multi-instance step engine (see sb_engine.h).
+#include "sb_engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <queue>
+
+using namespace sbk;
+
+#define HIP_TRY(call)                                          \
+    do {                                                       \
+        hipError_t _e = (call);                                \
+        if (_e != hipSuccess) return sbh::hip_fail(_e, #call); \
+    } while (0)
+
+namespace sbh {
+
+int engine_family(int precision, int variant, const sb_problem *prob)
+{
+    if (precision == SB_PREC_FP32) return FAM_FP32;
+    if (precision == SB_PREC_SPLIT) {
+        // Batched instances stream J from HBM every step: the FP8 form when
+        // exact halves the bytes of the BF16 form.
+        if (prob->Jfp8) return variant == SB_DSB ? FAM_DSB_FP8 : FAM_SPLIT_FP8;
+        return FAM_SPLIT_BF16;
+    }
+    return -1;
+}
+
+Engine::Engine(sb_ctx *ctx) : ctx_(ctx)
+{
+    grid_ = batch_grid_size(ctx->device);
+    const char *g = std::getenv("SB_GRAPH");
+    if (g && g[0] == '0') use_graph_ = false;
+}
+
+Engine::~Engine()
+{
+    (void)hipStreamSynchronize(ctx_->stream);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (size_t i = 0; i < slots_.size(); ++i)
+        if (slots_[i].used) remove((int)i);
+    (void)hipFree(d_inst_);
+    (void)hipFree(d_items_);
+    (void)hipFree(d_wg_);
+    if (h_inst_) (void)hipHostFree(h_inst_);
+}
+
+size_t Engine::state_bytes(int N, int R)
+{
+    const size_t e = (size_t)R * round_up(N, kSpinPad);
+    return e * sizeof(float) * 3 + e * 4 * 2;  // X[2], Y, planes[2] (<= 4 B/element)
+}
+
+int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
+{
+    const int N = prob->N, ld = prob->ld, R = p.replicas;
+    const int family = engine_family(p.precision, p.variant, prob);
+    if (family < 0) {
+        set_error("batched solve supports SB_PREC_FP32 and SB_PREC_SPLIT");
+        return SB_E_INVALID;
+    }
+    int bshape = 0;
+    int rc = choose_batch_shape(ctx_, prob, family, p.variant, R, &bshape);
+    if (rc) return rc;
+    int idx = -1;
+    for (size_t i = 0; i < slots_.size(); ++i)
+        if (!slots_[i].used) {
+            idx = (int)i;
+            break;
+        }
+    if (idx < 0) {
+        slots_.emplace_back();
+        idx = (int)slots_.size() - 1;
+    }
+    EngineSlot &s = slots_[idx];
+    s = EngineSlot();
+    s.prob = prob;
+    s.params = p;
+    s.c0 = c0;
+    s.family = family;
+    s.bshape = bshape;
+    s.bkind = plane_kind(family, p.variant);
+    const size_t e = (size_t)R * ld;
+    s.bytes = state_bytes(N, R);
+    hipError_t err = hipSuccess;
+    for (int b = 0; b < 2 && err == hipSuccess; ++b) err = hipMalloc((void **)&s.X[b], e * sizeof(float));
+    if (err == hipSuccess) err = hipMalloc((void **)&s.Y, e * sizeof(float));
+    if (err == hipSuccess && s.bkind != BK_NONE)
+        for (int b = 0; b < 2 && err == hipSuccess; ++b) err = hipMalloc((void **)&s.P[b], e * plane_bytes(s.bkind));
+    if (err != hipSuccess) {
+        s.used = true;
+        remove(idx);
+        return hip_fail(err, "hipMalloc(engine slot)");
+    }
+    s.used = true;
+    hipStream_t st = ctx_->stream;
+    HIP_TRY(launch_init(p.init, p.seed, N, R, ld, s.X[0], s.Y, s.X[1], st));
+    if (p.variant == SB_ASB) {
+        sb_step_coeffs c{};
+        c.a0 = p.a0;
+        c.K = p.K;
+        c.dts = p.dt / (double)p.M;
+        c.M = p.M;
+        c.p_next = 0.0f;
+        HIP_TRY(launch_asb_local(c, N, R, ld, s.X[0], s.Y, st));
+    }
+    // Both plane buffers start complete (the padding columns are read by the
+    // K loop; the epilogue rewrites only groups holding real spins).
+    if (s.bkind != BK_NONE) {
+        HIP_TRY(launch_make_planes(s.bkind, s.X[0], R, ld, s.P[0], st));
+        HIP_TRY(launch_make_planes(s.bkind, s.X[0], R, ld, s.P[1], st));
+    }
+    *slot_out = idx;
+    return SB_OK;
+}
+
+void Engine::remove(int slot)
+{
+    EngineSlot &s = slots_[slot];
+    if (!s.used) return;
+    (void)hipStreamSynchronize(ctx_->stream);
+    for (int b = 0; b < 2; ++b) {
+        (void)hipFree(s.X[b]);
+        (void)hipFree(s.P[b]);
+    }
+    (void)hipFree(s.Y);
+    s = EngineSlot();
+}
+
+int Engine::active_count() const
+{
+    int n = 0;
+    for (const auto &s : slots_)
+        if (s.used && s.steps_done < s.params.steps) ++n;
+    return n;
+}
+
+int Engine::ensure_tables(size_t n_slots, size_t n_items)
+{
+    bool realloc = false;
+    if (cap_inst_ < n_slots || !d_inst_) {
+        size_t cap = std::max<size_t>(64, std::max(n_slots, cap_inst_ * 2));
+        (void)hipStreamSynchronize(ctx_->stream);
+        (void)hipFree(d_inst_);
+        if (h_inst_) (void)hipHostFree(h_inst_);
+        d_inst_ = nullptr;
+        h_inst_ = nullptr;
+        HIP_TRY(hipMalloc((void **)&d_inst_, cap * sizeof(BatchInstance)));
+        HIP_TRY(hipHostMalloc((void **)&h_inst_, cap * sizeof(BatchInstance), hipHostMallocDefault));
+        std::memset(h_inst_, 0, cap * sizeof(BatchInstance));
+        cap_inst_ = cap;
+        realloc = true;
+    }
+    if (cap_items_ < n_items || !d_items_) {
+        size_t cap = std::max<size_t>(1024, std::max(n_items, cap_items_ * 2));
+        (void)hipStreamSynchronize(ctx_->stream);
+        (void)hipFree(d_items_);
+        d_items_ = nullptr;
+        HIP_TRY(hipMalloc((void **)&d_items_, cap * sizeof(BatchItem)));
+        cap_items_ = cap;
+        realloc = true;
+    }
+    if (!d_wg_) {
+        HIP_TRY(hipMalloc((void **)&d_wg_, (size_t)(grid_ + 1) * sizeof(int)));
+        realloc = true;
+    }
+    if (realloc && exec_) {
+        // Captured launches point at the old tables.
+        (void)hipGraphExecDestroy(exec_);
+        (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+        exec_C_ = 0;
+    }
+    return SB_OK;
+}
+
+// Work lists: instances are spread over the 8 XCDs (largest first onto the
+// least loaded XCD) so each instance's J and state stay in one XCD's L2; the
+// tiles of an XCD's instances then go, largest first, to that XCD's least
+// loaded workgroup (workgroup w runs on XCD w % 8).
+int Engine::rebuild()
+{
+    if (grid_ <= 0) {
+        set_error("batched kernel does not fit on the device");
+        return SB_E_HIP;
+    }
+    const size_t n = slots_.size();
+    int rc = ensure_tables(std::max<size_t>(n, 1), 1);
+    if (rc) return rc;
+    struct Job {
+        double cost;
+        int slot, tile;
+    };
+    std::vector<std::vector<Job>> per_slot(n);
+    std::vector<double> slot_cost(n, 0.0);
+    size_t total_items = 0;
+    for (size_t i = 0; i < n; ++i) {
+        EngineSlot &s = slots_[i];
+        BatchInstance &I = h_inst_[i];
+        std::memset(&I, 0, sizeof(I));
+        if (!s.used) continue;
+        const TileShape sh = batch_shape_at(s.family, s.bshape);
+        const int N = s.prob->N, R = s.params.replicas;
+        I.J = family_J(s.family, s.prob);
+        I.X[0] = s.X[0];
+        I.X[1] = s.X[1];
+        I.Y = s.Y;
+        I.P[0] = s.P[0];
+        I.P[1] = s.P[1];
+        I.bplane = (size_t)R * s.prob->ld;
+        I.bkind = s.bkind;
+        I.ldx = s.prob->ld;
+        I.N = N;
+        I.R = R;
+        I.family = s.family;
+        I.bshape = s.bshape;
+        I.tiles_m = (N + sh.mt() - 1) / sh.mt();
+        I.tiles_n = (R + sh.nt() - 1) / sh.nt();
+        I.steps = s.params.steps;
+        I.step_base = s.steps_done;
+        I.active = s.steps_done < s.params.steps ? 1 : 0;
+        I.variant = s.params.variant;
+        I.M = s.params.M;
+        I.dt = s.params.dt;
+        I.a0 = s.params.a0;
+        I.c0 = s.c0;
+        I.K = s.params.K;
+        if (!I.active) continue;
+        const double tile_cost = (double)sh.mt() * sh.nt() * (double)round_up(N, kSpinPad);
+        for (int t = 0; t < I.tiles_m * I.tiles_n; ++t) per_slot[i].push_back({tile_cost, (int)i, t});
+        slot_cost[i] = tile_cost * I.tiles_m * I.tiles_n;
+        total_items += per_slot[i].size();
+    }
+    rc = ensure_tables(std::max<size_t>(n, 1), std::max<size_t>(total_items, 1));
+    if (rc) return rc;
+
+    const int nx = 8;
+    std::vector<int> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return slot_cost[a] > slot_cost[b]; });
+    std::vector<double> xcd_load(nx, 0.0);
+    std::vector<std::vector<int>> xcd_slots(nx);
+    for (int i : order) {
+        if (per_slot[i].empty()) continue;
+        int x = (int)(std::min_element(xcd_load.begin(), xcd_load.end()) - xcd_load.begin());
+        xcd_load[x] += slot_cost[i];
+        xcd_slots[x].push_back(i);
+    }
+    std::vector<std::vector<BatchItem>> wg_items(grid_);
+    for (int x = 0; x < nx; ++x) {
+        std::vector<int> wgs;
+        for (int w = x; w < grid_; w += nx) wgs.push_back(w);
+        if (wgs.empty()) wgs.push_back(x % grid_);
+        std::vector<Job> jobs;
+        for (int i : xcd_slots[x]) jobs.insert(jobs.end(), per_slot[i].begin(), per_slot[i].end());
+        std::stable_sort(jobs.begin(), jobs.end(), [](const Job &a, const Job &b) { return a.cost > b.cost; });
+        typedef std::pair<double, int> LW;
+        std::priority_queue<LW, std::vector<LW>, std::greater<LW>> heap;
+        for (int w : wgs) heap.push({0.0, w});
+        for (const Job &j : jobs) {
+            LW top = heap.top();
+            heap.pop();
+            wg_items[top.second].push_back({j.slot, j.tile});
+            heap.push({top.first + j.cost, top.second});
+        }
+    }
+    std::vector<BatchItem> flat;
+    flat.reserve(total_items);
+    std::vector<int> begin(grid_ + 1, 0);
+    for (int w = 0; w < grid_; ++w) {
+        begin[w] = (int)flat.size();
+        flat.insert(flat.end(), wg_items[w].begin(), wg_items[w].end());
+    }
+    begin[grid_] = (int)flat.size();
+    n_items_ = (int)flat.size();
+    hipStream_t st = ctx_->stream;
+    if (!flat.empty())
+        HIP_TRY(hipMemcpyAsync(d_items_, flat.data(), flat.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_wg_, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_inst_, h_inst_, std::max<size_t>(n, 1) * sizeof(BatchInstance), hipMemcpyHostToDevice,
+                           st));
+    // The pageable sources above must stay valid until the copies complete.
+    HIP_TRY(hipStreamSynchronize(st));
+    return SB_OK;
+}
+
+int Engine::capture(int C)
+{
+    if (exec_ && exec_C_ == C) return SB_OK;
+    if (exec_) {
+        (void)hipGraphExecDestroy(exec_);
+        (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+    }
+    hipStream_t st = ctx_->stream;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < C && e == hipSuccess; ++c) e = launch_batch_step(grid_, d_inst_, d_items_, d_wg_, c, st);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e != hipSuccess ? e : e2, "graph capture");
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    graph_ = g;
+    exec_ = x;
+    exec_C_ = C;
+    return SB_OK;
+}
+
+void Engine::upload_bases(int C)
+{
+    (void)C;
+    for (size_t i = 0; i < slots_.size(); ++i) {
+        const EngineSlot &s = slots_[i];
+        h_inst_[i].step_base = s.steps_done;
+        h_inst_[i].active = (s.used && s.steps_done < s.params.steps) ? 1 : 0;
+    }
+}
+
+int Engine::run_chunk(int C)
+{
+    if (C < 1) return SB_E_INVALID;
+    hipStream_t st = ctx_->stream;
+    upload_bases(C);
+    const size_t n = std::max<size_t>(slots_.size(), 1);
+    HIP_TRY(hipMemcpyAsync(d_inst_, h_inst_, n * sizeof(BatchInstance), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ctx_->ev0, st));
+    graph_used_ = false;
+    if (use_graph_) {
+        int rc = capture(C);
+        if (rc == SB_OK) {
+            HIP_TRY(hipGraphLaunch(exec_, st));
+            graph_used_ = true;
+        }
+    }
+    if (!graph_used_)
+        for (int c = 0; c < C; ++c) HIP_TRY(launch_batch_step(grid_, d_inst_, d_items_, d_wg_, c, st));
+    HIP_TRY(hipEventRecord(ctx_->ev1, st));
+    HIP_TRY(hipEventSynchronize(ctx_->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx_->ev0, ctx_->ev1));
+    last_ms_ = ms;
+    for (auto &s : slots_)
+        if (s.used) s.steps_done = std::min(s.params.steps, s.steps_done + C);
+    return SB_OK;
+}
+
+int Engine::read(int slot, sb_result *res)
+{
+    EngineSlot &s = slots_[slot];
+    if (!s.used) return SB_E_INVALID;
+    const int R = s.params.replicas, N = s.prob->N;
+    int rc = ensure_workspace(ctx_, 0, R, N);
+    if (rc) return rc;
+    const int k = s.steps_done;
+    rc = read_results(ctx_, s.prob, s.X[k & 1], s.Y, R, res, ctx_->stream);
+    if (rc) return rc;
+    res->c0_used = s.c0;
+    return SB_OK;
+}
+
+}  // namespace sbh

@@ -103,37 +103,72 @@ hipError_t launch_pack_bf16(const float *J, size_t n_elems, uint16_t *out, hipSt
 hipError_t launch_pack_fp8(const float *J, size_t n_elems, uint8_t *out, hipStream_t s);
 
 // ---- batched launch ------------------------------------------------------
-// One work item = one (instance, row tile, replica tile) of one step.
+// Many instances advance one step per launch.  A work item is one tile
+// (row tile x replica tile) of one instance; each workgroup of the persistent
+// grid walks its own item list.  An instance's step index in a launch is
+// step_base + chunk_step, so one captured chunk of launches can be replayed
+// while the host moves step_base (and the instance set) between replays.
 struct BatchInstance {
-    const void *J;
-    const float *X0;   // ping-pong state buffers
-    float *X1;
+    const void *J;      // J in the family's format, [N][ldx]
+    float *X[2];        // ping-pong state, read X[k & 1], write X[(k + 1) & 1]
     float *Y;
-    int ldj, ldx;
+    void *P[2];         // B-operand planes (split families), same ping-pong
+    size_t bplane;
+    int bkind;
+    int ldx;
     int N, R;
-    int family, shape;  // kernel family and tile shape for this instance
+    int family, bshape;  // kernel family and batch tile shape
     int tiles_m, tiles_n;
-    int steps;          // schedule length S
-    int step_base;      // step index of the first launch of the current chunk
+    int steps;           // schedule length S
+    int step_base;       // absolute step of chunk_step 0
+    int active;          // 0: skip (slot free or finished)
     int variant, M;
-    float alpha_unused;
     double dt, a0, c0, K;
 };
 
 struct BatchItem {
-    int inst;   // index into the instance table
-    int tile;   // tile index within the instance (tm + tiles_m * tn)
+    int inst;  // index into the instance table
+    int tile;  // tile index within the instance (tm + tiles_m * tn)
 };
 
-struct BatchArgs {
-    const BatchInstance *inst;  // device table
-    const BatchItem *items;     // device list, grouped per workgroup
-    const int *wg_begin;        // [workgroups + 1] item range of each workgroup
-    int chunk_step;             // step offset inside the chunk (0..)
-    int workgroups;
-};
+// Tile shapes the batched kernel compiles per family (4 waves, <= 74 KB LDS).
+int batch_shape_count(int family);
+TileShape batch_shape_at(int family, int idx);
+size_t batch_lds_bytes();  // dynamic LDS of every batched launch
+// Index of a batch shape in the single-instance table of the same family.
+int batch_shape_solo_index(int family, int idx);
 
-// Persistent batched step: every workgroup walks its own item list.
-hipError_t launch_batch_step(int family, const BatchArgs &a, hipStream_t s);
+// One persistent launch of `grid` workgroups over the item lists.
+hipError_t launch_batch_step(int grid, const BatchInstance *insts, const BatchItem *items, const int *wg_begin,
+                             int chunk_step, hipStream_t s);
+// Workgroups of the batched kernel that fit on the device at once.
+int batch_grid_size(int device);
+
+// Step coefficients (identical arithmetic on host and device).
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+inline sb_step_coeffs step_coeffs(int variant, int k, int steps, double dt, double a0, double c0, double K, int M)
+{
+    sb_step_coeffs c{};
+    c.variant = variant;
+    c.M = M;
+    c.a0 = a0;
+    c.K = K;
+    c.dts = M > 0 ? dt / (double)M : 0.0;
+    if (variant == SB_ASB) {
+        c.beta = (float)(dt * c0);  // m2_opt.cu:148 coeff1 = DeltaT * xi
+        c.do_local = (k + 1 < steps) ? 1 : 0;
+        c.p_next = (k + 1 < steps) ? (float)(k + 1) / (float)steps : 0.0f;  // m2_opt.cu:158
+    } else {
+        double pk = (double)k / (double)steps;
+        double a = a0 * pk;
+        c.alpha = (float)(dt * (a0 - a));
+        c.beta = (float)(dt * c0);
+        c.gamma = (float)(dt * a0);
+        c.p_next = (float)pk;
+    }
+    return c;
+}
 
 }  // namespace sbk

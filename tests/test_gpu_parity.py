@@ -130,8 +130,29 @@ def test_dsb_integer_all_paths_identical(ctx, orc):
     sols = [ctx.solve(prob, state=True, variant="dsb", precision=p, steps=steps, replicas=R,
                       init="uniform", seed=2, dt=0.5) for p in ("fp32", "split", "ordered")]
     xo, yo, _ = oracle_run(orc, sb.DSB, J, R, steps, "uniform", 2, 0.5)
-    for s in sols:
-        assert np.array_equal(s.x, xo) and np.array_equal(s.y, yo)
+    for name, s in zip(("fp32", "split", "ordered"), sols):
+        bad = np.where(~(np.all(s.x == xo, axis=1) & np.all(s.y == yo, axis=1)))[0]
+        assert bad.size == 0, f"{name}: replicas {bad.tolist()} differ"
+
+
+def test_stale_workspace_planes_do_not_leak(orc):
+    # Regression: the second B-operand plane buffer was not initialised, so
+    # its padding columns (N..ld) kept bytes of an earlier, larger solve; J is
+    # zero there but 0 * NaN is NaN.  Leave NaNs behind with a diverging aSB
+    # solve at N=300, then solve dSB at N=260 (padding 260..319) on the same
+    # context and compare with the oracle.
+    with sb.Context(0) as c:
+        big = c.problem(kn_pm1(300, 90))
+        blown = c.solve(big, state=True, variant="asb", precision="split", steps=40, replicas=48,
+                        init="uniform", seed=1, dt=1.0, c0=1.0)
+        assert np.isnan(blown.x).any()
+        J = kn_pm1(260, 91)
+        small = c.problem(J)
+        xo, yo, _ = oracle_run(orc, sb.DSB, J, 48, 60, "uniform", 2, 0.5)
+        for prec in ("split", "fp32"):
+            s = c.solve(small, state=True, variant="dsb", precision=prec, steps=60, replicas=48, init="uniform",
+                        seed=2, dt=0.5)
+            assert np.array_equal(s.x, xo) and np.array_equal(s.y, yo), prec
 
 
 def test_results_independent_of_tile_and_batch_shape(ctx):

@@ -219,58 +219,16 @@ extern "C" SB_API double sb_c0_auto(int32_t variant, int32_t N, double a0, doubl
     return 0.5 / (std::sqrt((double)N) * sigma);
 }
 
-namespace {
-
-struct SplitMix {
-    uint64_t s;
-    uint64_t next()
-    {
-        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        return z ^ (z >> 31);
-    }
-    double uniform() { return (double)(next() >> 11) * 0x1p-53; }
-    double normal()
-    {
-        double u1 = uniform(), u2 = uniform();
-        if (u1 < 0x1p-60) u1 = 0x1p-60;
-        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-    }
-};
-
-}  // namespace
-
+// Keyed per-pair generator (sbk::gen_coupling): J_ij depends on (seed, i, j)
+// only, identical to the device generator of sb_problem_generate.
 extern "C" SB_API int sb_generate(int32_t kind, int32_t N, double p, uint64_t seed, float *J)
 {
     if (N < 1 || !J || kind < 0 || kind > 3) return SB_E_INVALID;
     if ((kind == 0 || kind == 3) && !(p >= 0.0 && p <= 1.0)) return SB_E_INVALID;
-    SplitMix rng{seed};
-    const double inv_sqrt_n = 1.0 / std::sqrt((double)N);
-    for (int i = 0; i < N; ++i) J[(size_t)i * N + i] = 0.0f;
     for (int i = 0; i < N; ++i) {
+        J[(size_t)i * N + i] = 0.0f;
         for (int j = i + 1; j < N; ++j) {
-            float v = 0.0f;
-            switch (kind) {
-            case 0: {  // Erdos-Renyi, w = +-1, J = -W
-                double u = rng.uniform();
-                uint64_t b = rng.next();
-                if (u < p) v = (b & 1) ? -1.0f : 1.0f;
-                break;
-            }
-            case 1:  // SK
-                v = (float)(rng.normal() * inv_sqrt_n);
-                break;
-            case 2:  // complete graph +-1
-                v = (rng.next() & 1) ? 1.0f : -1.0f;
-                break;
-            case 3: {  // diluted SK
-                double u = rng.uniform();
-                double z = rng.normal();
-                if (u < p) v = (float)(z * inv_sqrt_n);
-                break;
-            }
-            }
+            float v = gen_coupling(kind, N, p, seed, (uint32_t)i, (uint32_t)j);
             J[(size_t)i * N + j] = v;
             J[(size_t)j * N + i] = v;
         }
@@ -507,6 +465,79 @@ extern "C" SB_API int sb_problem_info(const sb_problem *p, int32_t *N, int32_t *
     if (is_integer) *is_integer = p->integer ? 1 : 0;
     if (sigma) *sigma = p->sigma;
     if (split_jfmt) *split_jfmt = p->Jfp8 ? SB_J_FP8 : SB_J_BF16;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_generate(sb_ctx *ctx, int32_t kind, int32_t N, double p, uint64_t seed,
+                                          sb_problem **out)
+{
+    if (!ctx || !out || N < 1 || kind < 0 || kind > 3) return SB_E_INVALID;
+    if ((kind == 0 || kind == 3) && !(p >= 0.0 && p <= 1.0)) return SB_E_INVALID;
+    *out = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    sb_problem *pr = new sb_problem();
+    pr->ctx = ctx;
+    pr->N = N;
+    pr->ld = round_up(N, kSpinPad);
+    const size_t elems = (size_t)N * pr->ld;
+    double *row_ss = nullptr;
+    float *row_max = nullptr;
+    int *row_int = nullptr;
+    std::vector<double> ss(N);
+    std::vector<float> mx(N);
+    std::vector<int> in(N);
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMalloc((void **)&pr->J32, elems * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&row_ss, N * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&row_max, N * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&row_int, N * sizeof(int));
+    if (e == hipSuccess) e = launch_generate(kind, N, p, seed, pr->J32, pr->ld, s);
+    if (e == hipSuccess) e = launch_row_stats(pr->J32, N, pr->ld, row_ss, row_max, row_int, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ss.data(), row_ss, N * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(mx.data(), row_max, N * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(in.data(), row_int, N * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    double tot = 0.0, m = 0.0;
+    bool integer = true;
+    for (int i = 0; i < N; ++i) {
+        tot += ss[i];
+        m = std::max(m, (double)mx[i]);
+        integer = integer && in[i] != 0;
+    }
+    pr->sigma = N > 1 ? std::sqrt(tot / ((double)N * (double)(N - 1))) : 0.0;
+    pr->max_abs = m;
+    pr->integer = integer;
+    if (e == hipSuccess) e = hipMalloc((void **)&pr->Jbf16, elems * sizeof(uint16_t));
+    if (e == hipSuccess) e = launch_pack_bf16(pr->J32, elems, pr->Jbf16, s);
+    if (e == hipSuccess && pr->integer && m <= 16.0) {
+        e = hipMalloc((void **)&pr->Jfp8, elems);
+        if (e == hipSuccess) e = launch_pack_fp8(pr->J32, elems, pr->Jfp8, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(row_ss);
+    (void)hipFree(row_max);
+    (void)hipFree(row_int);
+    if (e != hipSuccess) {
+        (void)hipFree(pr->J32);
+        (void)hipFree(pr->Jbf16);
+        (void)hipFree(pr->Jfp8);
+        delete pr;
+        return sbh::hip_fail(e, "sb_problem_generate");
+    }
+    ctx->live_problems++;
+    *out = pr;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_read_j(const sb_problem *p, float *J, int32_t ldj)
+{
+    if (!p || !J || ldj < p->N) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    HIP_TRY(hipMemcpy2DAsync(J, (size_t)ldj * sizeof(float), p->J32, (size_t)p->ld * sizeof(float),
+                             (size_t)p->N * sizeof(float), (size_t)p->N, hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
     return SB_OK;
 }
 

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "../../include/sb.h"
 
 namespace sbk {
@@ -143,6 +145,58 @@ hipError_t launch_batch_step(int grid, const BatchInstance *insts, const BatchIt
                              int chunk_step, hipStream_t s);
 // Workgroups of the batched kernel that fit on the device at once.
 int batch_grid_size(int device);
+
+// ---- synthetic instances ----------------------------------------------------
+// J_ij (i < j) is a function of (seed, i, j) only: a splitmix64 finaliser of
+// the pair key, so host and device generate the same instance in any order.
+// (SK values go through log/cos/sqrt; host and device libm may round those
+// differently in rare last bits, so read J back when the host needs it.)
+#ifdef __HIPCC__
+#define SB_HD __host__ __device__
+#else
+#define SB_HD
+#endif
+
+SB_HD inline uint64_t gen_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+SB_HD inline uint64_t gen_hash(uint64_t seed, uint32_t i, uint32_t j, uint32_t c)
+{
+    uint64_t key = ((uint64_t)i << 32) | (uint64_t)j;
+    return gen_mix(seed + 0x9E3779B97F4A7C15ull * (4ull * key + (uint64_t)c + 1ull));
+}
+
+SB_HD inline double gen_u01(uint64_t h) { return (double)(h >> 11) * 0x1p-53; }
+
+// kind: 0 ER(p) +-1 with J = -W, 1 SK N(0,1)/sqrt(N), 2 K_N +-1, 3 SK kept with prob. p
+SB_HD inline float gen_coupling(int kind, int N, double p, uint64_t seed, uint32_t i, uint32_t j)
+{
+    switch (kind) {
+    case 0:
+        if (gen_u01(gen_hash(seed, i, j, 0)) < p) return (gen_hash(seed, i, j, 1) & 1) ? 1.0f : -1.0f;
+        return 0.0f;
+    case 2: return (gen_hash(seed, i, j, 0) & 1) ? 1.0f : -1.0f;
+    case 1:
+    case 3: {
+        if (kind == 3 && !(gen_u01(gen_hash(seed, i, j, 2)) < p)) return 0.0f;
+        double u1 = gen_u01(gen_hash(seed, i, j, 0)), u2 = gen_u01(gen_hash(seed, i, j, 1));
+        if (u1 < 0x1p-60) u1 = 0x1p-60;
+        double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        return (float)(z / sqrt((double)N));
+    }
+    default: return 0.0f;
+    }
+}
+
+// Device generation of a dense J [N][ld] (zero diagonal and padding), and
+// per-row statistics: sum of squares (FP64), max |J|, integer flag.
+hipError_t launch_generate(int kind, int N, double p, uint64_t seed, float *J, int ld, hipStream_t s);
+hipError_t launch_row_stats(const float *J, int N, int ld, double *row_ss, float *row_max, int *row_int,
+                            hipStream_t s);
 
 // Step coefficients (identical arithmetic on host and device).
 #ifdef __HIPCC__

@@ -893,6 +893,74 @@ int batch_grid_size(int device)
     return cus * per_cu;
 }
 
+// ---------------------------------------------------------------------------
+// Synthetic instances on the device (gen_coupling in sb_internal.h).
+
+__global__ __launch_bounds__(256) void k_generate(int kind, int N, double p, uint64_t seed, float *J, int ld)
+{
+    const int i = blockIdx.y;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ld; j += gridDim.x * blockDim.x) {
+        float v = 0.0f;
+        if (j < N && j != i)
+            v = i < j ? gen_coupling(kind, N, p, seed, (uint32_t)i, (uint32_t)j)
+                      : gen_coupling(kind, N, p, seed, (uint32_t)j, (uint32_t)i);
+        J[(size_t)i * ld + j] = v;
+    }
+}
+
+hipError_t launch_generate(int kind, int N, double p, uint64_t seed, float *J, int ld, hipStream_t s)
+{
+    if (N <= 0) return hipSuccess;
+    int gx = (ld + 255) / 256;
+    hipLaunchKernelGGL(k_generate, dim3(gx < 16 ? gx : 16, N), dim3(256), 0, s, kind, N, p, seed, J, ld);
+    return hipGetLastError();
+}
+
+// One workgroup per row: sum of squares (FP64, fixed tree), max |J|, and
+// whether every entry is an integer.
+__global__ __launch_bounds__(256) void k_row_stats(const float *J, int N, int ld, double *row_ss, float *row_max,
+                                                   int *row_int)
+{
+    __shared__ double ss[256];
+    __shared__ float mx[256];
+    __shared__ int in[256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    double s = 0.0;
+    float m = 0.0f;
+    int integ = 1;
+    for (int j = t; j < N; j += 256) {
+        float v = J[(size_t)i * ld + j];
+        s += (double)v * (double)v;
+        m = fmaxf(m, fabsf(v));
+        if (v != floorf(v)) integ = 0;
+    }
+    ss[t] = s;
+    mx[t] = m;
+    in[t] = integ;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] += ss[t + w];
+            mx[t] = fmaxf(mx[t], mx[t + w]);
+            in[t] &= in[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        row_ss[i] = ss[0];
+        row_max[i] = mx[0];
+        row_int[i] = in[0];
+    }
+}
+
+hipError_t launch_row_stats(const float *J, int N, int ld, double *row_ss, float *row_max, int *row_int,
+                            hipStream_t s)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_stats, dim3(N), dim3(256), 0, s, J, N, ld, row_ss, row_max, row_int);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_make_planes(int kind, const float *X, size_t n4, void *B, size_t bplane)
 {
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (size_t)gridDim.x * blockDim.x) {

@@ -34,10 +34,12 @@ EXPORTS = (
     "sb_version", "sb_status_string", "sb_last_error_message", "sb_default_params",
     "sb_edgelist_parse", "sb_instance_stats", "sb_c0_auto", "sb_generate",
     "sb_ctx_create", "sb_ctx_destroy", "sb_problem_create",
-    "sb_problem_from_edgelist_file", "sb_problem_destroy", "sb_problem_info",
+    "sb_problem_from_edgelist_file", "sb_problem_generate", "sb_problem_read_j",
+    "sb_problem_destroy", "sb_problem_info",
     "sb_solve", "sb_step_coeffs_for", "sb_step", "sb_init_state", "sb_asb_prologue",
     "sb_energy", "sb_batch_solve", "sb_server_create", "sb_server_submit",
-    "sb_server_wait", "sb_server_cancel", "sb_server_stats_get", "sb_server_destroy",
+    "sb_server_submit_generated", "sb_server_wait", "sb_server_cancel", "sb_server_stats_get",
+    "sb_server_destroy",
 )
 
 
@@ -98,7 +100,8 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
         ("cancelled", C.c_int64), ("active", C.c_int64), ("queued", C.c_int64),
-        ("bytes_in_use", C.c_size_t), ("busy_ms", C.c_double),
+        ("bytes_in_use", C.c_size_t), ("busy_ms", C.c_double), ("chunks", C.c_int64),
+        ("max_active_seen", C.c_int64),
     ]
 
 
@@ -120,6 +123,8 @@ def _bind(lib: C.CDLL) -> None:
         "sb_ctx_destroy": (C.c_int, [P]),
         "sb_problem_create": (C.c_int, [P, I32, P, I32, C.POINTER(P)]),
         "sb_problem_from_edgelist_file": (C.c_int, [P, C.c_char_p, C.c_int, C.POINTER(P)]),
+        "sb_problem_generate": (C.c_int, [P, I32, I32, D, U64, C.POINTER(P)]),
+        "sb_problem_read_j": (C.c_int, [P, P, I32]),
         "sb_problem_destroy": (C.c_int, [P]),
         "sb_problem_info": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(D), C.POINTER(I32)]),
         "sb_solve": (C.c_int, [P, P, C.POINTER(Params), C.POINTER(Result)]),
@@ -131,6 +136,7 @@ def _bind(lib: C.CDLL) -> None:
         "sb_batch_solve": (C.c_int, [P, I32, P, C.POINTER(Params), C.POINTER(Result), C.POINTER(BatchStats)]),
         "sb_server_create": (C.c_int, [P, C.POINTER(ServerOpts), C.POINTER(P)]),
         "sb_server_submit": (C.c_int, [P, I32, P, I32, C.POINTER(Params), C.POINTER(U64)]),
+        "sb_server_submit_generated": (C.c_int, [P, I32, I32, D, U64, C.POINTER(Params), C.POINTER(U64)]),
         "sb_server_wait": (C.c_int, [P, U64, C.POINTER(Result), D]),
         "sb_server_cancel": (C.c_int, [P, U64]),
         "sb_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
@@ -276,6 +282,11 @@ class Problem:
         self.sigma = sigma.value
         self.split_jfmt = jf.value
 
+    def read_j(self) -> np.ndarray:
+        J = np.empty((self.N, self.N), dtype=np.float32)
+        _check(load().sb_problem_read_j(self.handle, _ptr(J), self.N), "sb_problem_read_j")
+        return J
+
     def close(self) -> None:
         # A problem must be destroyed before its context.
         if self.handle and self.ctx.handle:
@@ -325,6 +336,13 @@ class Context:
                "sb_problem_create")
         return Problem(self, h)
 
+    def generate(self, kind: int, N: int, p: float = 1.0, seed: int = 0) -> Problem:
+        """Synthetic instance generated on the device (see sb_generate)."""
+        h = C.c_void_p()
+        _check(self.lib.sb_problem_generate(self.handle, kind, N, float(p), int(seed), C.byref(h)),
+               "sb_problem_generate")
+        return Problem(self, h)
+
     def problem_from_file(self, path: str, one_based: bool = False) -> Problem:
         h = C.c_void_p()
         _check(self.lib.sb_problem_from_edgelist_file(self.handle, path.encode(), int(one_based), C.byref(h)),
@@ -350,3 +368,70 @@ class Context:
         st = BatchStats()
         _check(self.lib.sb_batch_solve(self.handle, n, arr_p, arr_par, arr_res, C.byref(st)), "sb_batch_solve")
         return [_solution(arr_res[i], keep[i]) for i in range(n)], st
+
+
+class Server:
+    """Continuous-batching request server (sb_server_* of include/sb.h)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, chunk_steps: int = 50, max_active: int = 0,
+                 precision="split"):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        o = ServerOpts()
+        o.hbm_budget_bytes = int(hbm_budget_bytes)
+        o.chunk_steps = chunk_steps
+        o.max_active = max_active
+        o.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        h = C.c_void_p()
+        _check(self.lib.sb_server_create(ctx.handle, C.byref(o), C.byref(h)), "sb_server_create")
+        self.handle = h
+        self._shape = {}
+
+    def submit(self, J: np.ndarray, **kw) -> int:
+        J = np.ascontiguousarray(J, dtype=np.float32)
+        p = make_params(**kw)
+        t = C.c_uint64()
+        _check(self.lib.sb_server_submit(self.handle, J.shape[0], _ptr(J), J.shape[1], C.byref(p), C.byref(t)),
+               "sb_server_submit")
+        self._shape[t.value] = (p.replicas, J.shape[0])
+        return t.value
+
+    def submit_generated(self, kind: int, N: int, p: float, gen_seed: int, **kw) -> int:
+        """Instance (kind, N, p, gen_seed) generated on the device; kw are solver params."""
+        par = make_params(**kw)
+        t = C.c_uint64()
+        _check(self.lib.sb_server_submit_generated(self.handle, kind, N, float(p), int(gen_seed), C.byref(par),
+                                                   C.byref(t)), "sb_server_submit_generated")
+        self._shape[t.value] = (par.replicas, N)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0, spins: bool = True) -> Solution:
+        R, N = self._shape.get(ticket, (1, 1))
+        res, bufs = _result_struct(R, N, spins, False)
+        try:
+            _check(self.lib.sb_server_wait(self.handle, ticket, C.byref(res), float(timeout_ms)), "sb_server_wait")
+        except SBError as e:
+            if e.status != 10:  # the ticket stays alive only after a timeout
+                self._shape.pop(ticket, None)
+            raise
+        self._shape.pop(ticket, None)
+        return _solution(res, bufs)
+
+    def cancel(self, ticket: int) -> None:
+        _check(self.lib.sb_server_cancel(self.handle, ticket), "sb_server_cancel")
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(self.lib.sb_server_stats_get(self.handle, C.byref(st)), "sb_server_stats_get")
+        return st
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sb_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

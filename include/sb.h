@@ -164,6 +164,13 @@ SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, int32_t ldj
                              sb_problem **out);
 SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *path, int one_based,
                                          sb_problem **out);
+/* Generate a synthetic instance on the device (kinds as sb_generate; the
+ * same J as sb_generate gives on the host, up to rare last-bit differences of
+ * libm in the SK kinds). */
+SB_API int sb_problem_generate(sb_ctx *ctx, int32_t kind, int32_t N, double p, uint64_t seed,
+                               sb_problem **out);
+/* Copy the problem's FP32 J to a host buffer [N][ldj]. */
+SB_API int sb_problem_read_j(const sb_problem *prob, float *J, int32_t ldj);
 SB_API int sb_problem_destroy(sb_problem *prob);
 SB_API int sb_problem_info(const sb_problem *prob, int32_t *N, int32_t *is_integer,
                            double *sigma, int32_t *split_jfmt);
@@ -240,6 +247,8 @@ typedef struct sb_server_stats {
     int64_t active, queued;
     size_t bytes_in_use;
     double busy_ms;           /* device time spent in step launches */
+    int64_t chunks;           /* chunks (graph replays) run            */
+    int64_t max_active_seen;  /* most requests resident at once        */
 } sb_server_stats;
 
 SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, sb_server **out);
@@ -247,8 +256,14 @@ SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, sb_server *
  * with SB_E_BUDGET when the request alone exceeds the budget. */
 SB_API int sb_server_submit(sb_server *srv, int32_t N, const float *J, int32_t ldj,
                             const sb_params *params, uint64_t *ticket);
-/* Wait for a ticket (timeout_ms < 0: forever).  Fills result like sb_solve
- * (spins/energies buffers sized for that request) and frees the ticket. */
+/* Submit a synthetic instance (sb_generate kinds) that the server generates
+ * on the device at admission; no host copy of J is kept. */
+SB_API int sb_server_submit_generated(sb_server *srv, int32_t kind, int32_t N, double p, uint64_t seed,
+                                      const sb_params *params, uint64_t *ticket);
+/* Wait for a ticket (timeout_ms < 0: forever).  Fills energy_f64 / energy_i64
+ * / spins of result (x_final, y_final are not kept by the server) and the
+ * best replica, then frees the ticket.  result->total_ms is the request's
+ * latency from submission to completion. */
 SB_API int sb_server_wait(sb_server *srv, uint64_t ticket, sb_result *result,
                           double timeout_ms);
 SB_API int sb_server_cancel(sb_server *srv, uint64_t ticket);

@@ -1,0 +1,117 @@
+"""Continuous-batching server: a request's result is the same alone, in a
+mix of concurrent requests, and from sb_solve; HBM-budget admission; cancel
+and timeouts; device instance generation."""
+import time
+
+import numpy as np
+import pytest
+
+from annealing_amd import sb
+
+pytestmark = pytest.mark.gpu
+
+GB = 1 << 30
+
+# (kind, N, p, seed, variant, R, steps)
+REQS = [
+    (sb.GEN_ER, 300, 0.3, 1, "bsb", 32, 120),
+    (sb.GEN_SK_DILUTED, 500, 0.5, 2, "bsb", 64, 200),
+    (sb.GEN_KN, 128, 1.0, 3, "dsb", 16, 80),
+    (sb.GEN_ER, 900, 0.05, 4, "dsb", 64, 150),
+    (sb.GEN_SK, 200, 1.0, 5, "asb", 8, 60),
+    (sb.GEN_KN, 1000, 1.0, 6, "bsb", 128, 100),
+]
+
+
+def submit(srv, spec, precision):
+    kind, N, p, seed, v, R, steps = spec
+    return srv.submit_generated(kind, N, p, seed, variant=v, precision=precision, steps=steps, replicas=R,
+                                init="uniform", seed=seed, dt=1.0 if v == "asb" else 0.5)
+
+
+@pytest.mark.parametrize("precision", ["split", "fp32"])
+def test_same_result_alone_and_in_mix(precision):
+    with sb.Context(0) as ctx, sb.Server(ctx, 8 * GB, chunk_steps=25, precision=precision) as srv:
+        alone = []
+        for spec in REQS:
+            alone.append(srv.wait(submit(srv, spec, precision)))
+        tickets = [submit(srv, spec, precision) for spec in REQS]
+        mixed = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        assert st.max_active_seen >= 2 and st.completed == 2 * len(REQS)
+        for i, (a, m) in enumerate(zip(alone, mixed)):
+            assert np.array_equal(a.energy, m.energy), i
+            assert np.array_equal(a.spins, m.spins), i
+        # and equal to sb_solve on the same (device-generated) instance
+        for i, spec in enumerate(REQS):
+            kind, N, p, seed, v, R, steps = spec
+            prob = ctx.generate(kind, N, p, seed)
+            s = ctx.solve(prob, variant=v, precision=precision, steps=steps, replicas=R, init="uniform", seed=seed,
+                          dt=1.0 if v == "asb" else 0.5)
+            assert np.array_equal(s.spins, alone[i].spins), i
+            assert np.array_equal(s.energy, alone[i].energy), i
+            prob.close()
+
+
+def test_host_j_submit_matches_solve(orc):
+    J = sb.generate(sb.GEN_ER, 333, 0.2, 9)
+    with sb.Context(0) as ctx, sb.Server(ctx, 4 * GB, chunk_steps=10) as srv:
+        t = srv.submit(J, variant="bsb", precision="split", steps=70, replicas=24, init="uniform", seed=1, dt=0.5)
+        r = srv.wait(t)
+        assert r.exact and np.array_equal(r.energy_i64, orc.energy_i64(J, r.spins))
+        prob = ctx.problem(J)
+        s = ctx.solve(prob, variant="bsb", precision="split", steps=70, replicas=24, init="uniform", seed=1, dt=0.5)
+        assert np.array_equal(s.spins, r.spins)
+        prob.close()
+
+
+def test_budget_rejection_and_backfill():
+    N, R = 1000, 64
+    per_req = N * 1024 * 7 + R * 1024 * 20  # J in three formats + state
+    with sb.Context(0) as ctx, sb.Server(ctx, int(2.5 * per_req), chunk_steps=20) as srv:
+        with pytest.raises(sb.SBError) as e:
+            srv.submit_generated(sb.GEN_KN, 4000, 1.0, 1, variant="bsb", precision="split", steps=10, replicas=64)
+        assert e.value.status == 8
+        tickets = [srv.submit_generated(sb.GEN_KN, N, 1.0, s, variant="bsb", precision="split", steps=60,
+                                        replicas=R, seed=s, dt=0.5) for s in range(5)]
+        for t in tickets:
+            srv.wait(t)
+        st = srv.stats()
+        assert st.rejected == 1 and st.completed == 5
+        assert 1 <= st.max_active_seen <= 2
+
+
+def test_cancel_and_timeout():
+    with sb.Context(0) as ctx, sb.Server(ctx, 8 * GB, chunk_steps=5) as srv:
+        long_t = srv.submit_generated(sb.GEN_KN, 2000, 1.0, 1, variant="bsb", precision="split", steps=200000,
+                                      replicas=64, seed=1, dt=0.5)
+        with pytest.raises(sb.SBError) as e:
+            srv.wait(long_t, timeout_ms=50)
+        assert e.value.status == 10
+        srv.cancel(long_t)
+        with pytest.raises(sb.SBError) as e:
+            srv.wait(long_t, timeout_ms=60000)
+        assert e.value.status == 11
+        t = srv.submit_generated(sb.GEN_ER, 100, 0.5, 2, variant="bsb", precision="split", steps=30, replicas=8)
+        assert srv.wait(t, timeout_ms=60000).energy.shape == (8,)
+        with pytest.raises(sb.SBError) as e:
+            srv.wait(12345)
+        assert e.value.status == 12
+
+
+@pytest.mark.parametrize("kind,p", [(sb.GEN_ER, 0.05), (sb.GEN_KN, 1.0), (sb.GEN_SK, 1.0), (sb.GEN_SK_DILUTED, 0.3)])
+def test_device_generation_matches_host(kind, p):
+    N = 700
+    Jh = sb.generate(kind, N, p, 77)
+    with sb.Context(0) as ctx:
+        prob = ctx.generate(kind, N, p, 77)
+        Jd = prob.read_j()
+        sigma, integer, _ = sb.instance_stats(Jh)
+        assert prob.integer == integer
+        assert abs(prob.sigma - sigma) <= 1e-6 * sigma
+        prob.close()
+    if kind in (sb.GEN_ER, sb.GEN_KN):
+        assert np.array_equal(Jd, Jh)
+    else:
+        # libm rounding may differ in rare last bits
+        assert np.mean(Jd == Jh) > 0.999 and np.allclose(Jd, Jh, rtol=1e-6, atol=0)

@@ -916,6 +916,60 @@ extern "C" SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, in
     return SB_OK;
 }
 
+extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision, int32_t variant, const float *X,
+                               int32_t R, float *F)
+{
+    if (!ctx || !prob || !X || !F || R < 1 || prob->ctx != ctx) return SB_E_INVALID;
+    if (variant < SB_ASB || variant > SB_DSB) return SB_E_INVALID;
+    if (precision < SB_PREC_FP32 || precision > SB_PREC_ORDERED) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int N = prob->N, ld = prob->ld;
+    int rc = sbh::ensure_workspace(ctx, (size_t)R * ld, R, N);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    // X into the padded workspace, Y = 0; one step with alpha = gamma = 0,
+    // beta = 1 leaves y' = fmaf(1, F, fmaf(-0, x, 0)) = F (bSB/dSB, |x| <= 1
+    // so no wall) or y' = fmaf(1, F, 0) = F (aSB without local sub-steps).
+    HIP_TRY(hipMemsetAsync(ctx->X0, 0, (size_t)R * ld * sizeof(float), s));
+    HIP_TRY(hipMemsetAsync(ctx->X1, 0, (size_t)R * ld * sizeof(float), s));
+    HIP_TRY(hipMemsetAsync(ctx->Y, 0, (size_t)R * ld * sizeof(float), s));
+    HIP_TRY(hipMemcpy2DAsync(ctx->X0, (size_t)ld * sizeof(float), X, (size_t)N * sizeof(float),
+                             (size_t)N * sizeof(float), (size_t)R, hipMemcpyHostToDevice, s));
+    StepArgs a{};
+    a.X = ctx->X0;
+    a.Xn = ctx->X1;
+    a.Y = ctx->Y;
+    a.ldj = ld;
+    a.ldx = ld;
+    a.N = N;
+    a.R = R;
+    a.c.variant = variant;
+    a.c.beta = 1.0f;
+    a.c.do_local = 0;
+    hipError_t e;
+    if (precision == SB_PREC_ORDERED) {
+        a.J = prob->J32;
+        e = launch_step_ordered(a, s);
+    } else {
+        int family = 0, shape = 0;
+        rc = sbh::choose_kernel(ctx, prob, precision, a.c.variant, R, &family, &shape);
+        if (rc) return rc;
+        // choose_kernel may have used the workspace: restore X and Y.
+        HIP_TRY(hipMemsetAsync(ctx->X0, 0, (size_t)R * ld * sizeof(float), s));
+        HIP_TRY(hipMemsetAsync(ctx->Y, 0, (size_t)R * ld * sizeof(float), s));
+        HIP_TRY(hipMemcpy2DAsync(ctx->X0, (size_t)ld * sizeof(float), X, (size_t)N * sizeof(float),
+                                 (size_t)N * sizeof(float), (size_t)R, hipMemcpyHostToDevice, s));
+        a.J = sbh::family_J(family, prob);
+        e = launch_step(family, shape, a, s);  // B staged from X (no planes)
+    }
+    if (e != hipSuccess) return sbh::hip_fail(e, "sb_field");
+    HIP_TRY(hipMemcpy2DAsync(F, (size_t)N * sizeof(float), ctx->Y, (size_t)ld * sizeof(float),
+                             (size_t)N * sizeof(float), (size_t)R, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SB_OK;
+}
+
 extern "C" SB_API int sb_init_state(sb_ctx *ctx, int32_t init, uint64_t seed, int32_t N, int32_t R, int32_t ldx,
                                     float *dX, float *dY, void *stream)
 {

@@ -36,7 +36,7 @@ EXPORTS = (
     "sb_ctx_create", "sb_ctx_destroy", "sb_problem_create",
     "sb_problem_from_edgelist_file", "sb_problem_generate", "sb_problem_read_j",
     "sb_problem_destroy", "sb_problem_info",
-    "sb_solve", "sb_step_coeffs_for", "sb_step", "sb_init_state", "sb_asb_prologue",
+    "sb_solve", "sb_step_coeffs_for", "sb_step", "sb_field", "sb_init_state", "sb_asb_prologue",
     "sb_energy", "sb_batch_solve", "sb_server_create", "sb_server_submit",
     "sb_server_submit_generated", "sb_server_wait", "sb_server_cancel", "sb_server_stats_get",
     "sb_server_destroy",
@@ -130,6 +130,7 @@ def _bind(lib: C.CDLL) -> None:
         "sb_solve": (C.c_int, [P, P, C.POINTER(Params), C.POINTER(Result)]),
         "sb_step_coeffs_for": (C.c_int, [C.POINTER(Params), D, I32, C.POINTER(StepCoeffs)]),
         "sb_step": (C.c_int, [P, I32, P, I32, I32, I32, P, P, P, I32, I32, C.POINTER(StepCoeffs), P]),
+        "sb_field": (C.c_int, [P, P, I32, I32, P, I32, P]),
         "sb_init_state": (C.c_int, [P, I32, U64, I32, I32, I32, P, P, P]),
         "sb_asb_prologue": (C.c_int, [P, C.POINTER(Params), I32, I32, I32, P, P, P]),
         "sb_energy": (C.c_int, [P, P, I32, I32, P, I32, I32, P, P]),
@@ -354,6 +355,16 @@ class Context:
         res, bufs = _result_struct(params.replicas, prob.N, spins, state)
         _check(self.lib.sb_solve(self.handle, prob.handle, C.byref(params), C.byref(res)), "sb_solve")
         return _solution(res, bufs)
+
+    def field(self, prob: Problem, X: np.ndarray, precision="fp32", variant="bsb") -> np.ndarray:
+        """F = J.X (or J.sign(X) for dSB) through the product kernels; X is [R, N]."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        R = X.shape[0]
+        F = np.zeros((R, prob.N), dtype=np.float32)
+        prec = PRECISIONS[precision] if isinstance(precision, str) else precision
+        var = VARIANTS[variant] if isinstance(variant, str) else variant
+        _check(self.lib.sb_field(self.handle, prob.handle, prec, var, _ptr(X), R, _ptr(F)), "sb_field")
+        return F
 
     def batch_solve(self, probs, params_list, spins: bool = True, state: bool = False):
         n = len(probs)

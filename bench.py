@@ -1,19 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark: replica spin-updates/s of bSB on SK N=2000, R=1024 (FP32 plain path).
 
-One bench step = one complete solve of the workload (keyed init, 2000 fused SB
+One bench step = one complete solve of that workload (keyed init, 2000 fused SB
 steps over 1024 replicas, exact energies and best-replica readout), with J
 already resident in HBM.  value = N * R * sb_steps * K / wall seconds of the K
-timed solves.  Extra legs (same JSON line): the precision form on K2000 +-1 at
-R=4096 (BF16 hi/lo split and FP8, vs FP32 at the same R), the roofline of the
-fused step kernel from HIP events, and the CPU oracle on a bounded sample.
+timed solves.
+
+Further legs on the same JSON line (each can be skipped):
+  precision_form  K2000 +-1, R=4096: BF16 hi/lo split and FP8 (dSB) vs FP32
+  batched         512 ER instances, N ~ U[64,4096], R=64, 1500 steps, one
+                  persistent launch per step replayed from a hipGraph
+  serving         Poisson arrivals at 200 req/s (window --serve-seconds), N
+                  log-uniform [100,8000], SK/+-1 halves, density U[0.01,1],
+                  R in {32,64,128,256}, steps U[1000,5000]: p50/p99 latency
+  accuracy        field error of the FP32 and split paths against an FP64
+                  trajectory (K2000, 8 replicas) and the final-energy gap
+  cpu_baseline    the CPU oracle on a bounded sample of the headline workload
 """
 import argparse
 import json
+import math
 import os
 import platform
 import sys
+import threading
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -22,6 +35,11 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP8_TFLOPS = 5000.0
 PEAK_HBM_GBS = 8000.0
+GB = 1 << 30
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_model() -> str:
@@ -45,6 +63,191 @@ def solve_timed(ctx, prob, K, W, **kw):
     return wall, last
 
 
+def leg_precision(sb, ctx, S):
+    N3, R3 = 2000, 4096
+    p3 = ctx.problem(sb.generate(sb.GEN_KN, N3, 1.0, seed=3))
+    leg = {}
+    for name, kw3 in (("fp32_bsb", dict(variant="bsb", precision="fp32")),
+                      ("split_bsb", dict(variant="bsb", precision="split")),
+                      ("fp32_dsb", dict(variant="dsb", precision="fp32")),
+                      ("split_dsb", dict(variant="dsb", precision="split"))):
+        common = dict(steps=S, replicas=R3, init="uniform", seed=3, dt=1.0)
+        w3, r3 = solve_timed(ctx, p3, 1, 1, **kw3, **common)
+        us = r3.step_ms * 1e3 / S
+        flops = 2.0 * N3 * N3 * R3
+        leg[name] = {"spin_updates_per_s": N3 * R3 * S / w3, "solve_s": w3, "avg_launch_us": us,
+                     "alg_tflops": flops / us / 1e6, "mean_energy": float(r3.energy.mean()),
+                     "best_energy": float(r3.energy.min())}
+        log(f"{name}: {leg[name]['spin_updates_per_s']:.4g} spin-updates/s, {us:.1f} us/launch")
+    f, s = leg["fp32_bsb"], leg["split_bsb"]
+    leg["speedup_split_bsb_vs_fp32"] = s["spin_updates_per_s"] / f["spin_updates_per_s"]
+    leg["speedup_split_dsb_vs_fp32"] = leg["split_dsb"]["spin_updates_per_s"] / leg["fp32_dsb"]["spin_updates_per_s"]
+    leg["mean_energy_gap_split_vs_fp32_bsb"] = (s["mean_energy"] - f["mean_energy"]) / abs(f["mean_energy"])
+    # split bSB issues two BF16 MFMA passes: 2 x the algorithmic flops
+    leg["split_bsb_roofline"] = {"bound": "mfma", "achieved_issued": 2 * s["alg_tflops"], "peak": PEAK_BF16_TFLOPS,
+                                 "unit": "TFLOP/s", "frac": 2 * s["alg_tflops"] / PEAK_BF16_TFLOPS}
+    # dSB FP8: state traffic per step, 16 B/spin-replica (x, y read+write) + 1 B sign plane written
+    st_bytes = (16 + 1) * N3 * R3 + N3 * N3
+    gbs = st_bytes / (leg["split_dsb"]["avg_launch_us"] * 1e-6) / 1e9
+    leg["split_dsb_roofline"] = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                 "frac": gbs / PEAK_HBM_GBS, "bytes_per_launch": st_bytes}
+    leg["workload"] = "K2000 +-1 (FP8-exact J), R=4096, 2000 steps, dt=1.0, seed 3"
+    p3.close()
+    return leg
+
+
+def leg_batched(sb, ctx, n_inst, steps):
+    rng = np.random.default_rng(4)
+    Ns = rng.integers(64, 4097, n_inst)
+    t0 = time.perf_counter()
+    probs = [ctx.generate(sb.GEN_ER, int(n), 0.05, 1000 + i) for i, n in enumerate(Ns)]
+    gen_s = time.perf_counter() - t0
+    leg = {"workload": f"{n_inst} ER instances N~U[64,4096] p=0.05 +-1, R=64, {steps} steps, bSB, seed 4",
+           "instances": n_inst, "sum_N": int(Ns.sum()), "generate_s": gen_s}
+    work = float(Ns.sum()) * 64 * steps
+    means = {}
+    for prec in ("fp32", "split"):
+        params = [sb.make_params(variant="bsb", precision=prec, steps=steps, replicas=64, init="uniform",
+                                 seed=i, dt=1.0) for i in range(n_inst)]
+        # warm-up (autotune of the (N, R) buckets) on a short run
+        warm = [sb.make_params(variant="bsb", precision=prec, steps=2, replicas=64, init="uniform", seed=i)
+                for i in range(n_inst)]
+        ctx.batch_solve(probs, warm, spins=False)
+        t0 = time.perf_counter()
+        sols, st = ctx.batch_solve(probs, params, spins=False)
+        wall = time.perf_counter() - t0
+        means[prec] = float(np.mean([s.energy.mean() for s in sols]))
+        leg[prec] = {"spin_updates_per_s": work / wall, "wall_s": wall, "device_step_ms": st.step_ms,
+                     "avg_launch_us": st.step_ms * 1e3 / steps, "work_items": st.work_items,
+                     "workgroups": st.workgroups, "graph_used": bool(st.graph_used)}
+        log(f"batched {prec}: {work / wall:.4g} spin-updates/s ({wall:.2f} s, {st.work_items} tiles/step)")
+    leg["speedup_split_vs_fp32"] = leg["split"]["spin_updates_per_s"] / leg["fp32"]["spin_updates_per_s"]
+    leg["mean_energy_gap_split_vs_fp32"] = (means["split"] - means["fp32"]) / abs(means["fp32"])
+    # J bytes streamed per step (FP8 split vs FP32)
+    jb = float((Ns.astype(np.float64) ** 2).sum())
+    leg["split"]["hbm_gbs_j_only"] = jb / (leg["split"]["avg_launch_us"] * 1e-6) / 1e9
+    leg["fp32"]["hbm_gbs_j_only"] = 4 * jb / (leg["fp32"]["avg_launch_us"] * 1e-6) / 1e9
+    for p in probs:
+        p.close()
+    return leg
+
+
+def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
+    rng = np.random.default_rng(5)
+    n_req = int(rng.poisson(rate * seconds))
+    arrivals = np.sort(rng.uniform(0.0, seconds, n_req))
+    Ns = np.exp(rng.uniform(math.log(100), math.log(8000), n_req)).astype(int)
+    kinds = np.where(rng.random(n_req) < 0.5, sb.GEN_SK_DILUTED, sb.GEN_ER)
+    dens = rng.uniform(0.01, 1.0, n_req)
+    Rs = rng.choice([32, 64, 128, 256], n_req)
+    steps = rng.integers(1000, 5001, n_req)
+    srv = sb.Server(ctx, hbm_budget_bytes=128 * GB, chunk_steps=50, precision="split")
+    tickets = [None] * n_req
+    lat = {}
+    done_evt = threading.Event()
+    lock = threading.Lock()
+
+    def waiter():
+        for i in range(n_req):
+            while tickets[i] is None:
+                time.sleep(0.001)
+            try:
+                r = srv.wait(tickets[i], spins=False)
+                with lock:
+                    lat[i] = r.total_ms
+            except sb.SBError:
+                pass
+        done_evt.set()
+
+    th = threading.Thread(target=waiter, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(n_req):
+        delay = arrivals[i] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets[i] = srv.submit_generated(int(kinds[i]), int(Ns[i]), float(dens[i]), 5000 + i, variant="bsb",
+                                          precision="split", steps=int(steps[i]), replicas=int(Rs[i]),
+                                          init="uniform", seed=i, dt=1.0)
+    submit_s = time.perf_counter() - t0
+    finished = done_evt.wait(timeout=drain_cap_s)
+    if not finished:
+        for i in range(n_req):
+            if i not in lat:
+                try:
+                    srv.cancel(tickets[i])
+                except sb.SBError:
+                    pass
+        done_evt.wait(timeout=120)
+    total_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    done_idx = sorted(lat)
+    L = np.array([lat[i] for i in done_idx]) if done_idx else np.zeros(1)
+    work_all = (Ns.astype(np.float64) ** 2 * 2 * Rs * steps)
+    work_done = float(work_all[done_idx].sum()) if done_idx else 0.0
+    spin_upd = float((Ns * Rs * steps)[done_idx].sum()) if done_idx else 0.0
+    leg = {
+        "workload": f"Poisson {rate} req/s for {seconds} s (seed 5): N log-uniform [100,8000], half diluted SK "
+                    "(BF16 J) half ER +-1 (FP8 J), density U[0.01,1], R in {32,64,128,256}, steps U[1000,5000], "
+                    "bSB split precision, chunk 50 steps, HBM budget 128 GiB",
+        "requests": n_req, "completed": len(done_idx), "cancelled_at_drain_cap": n_req - len(done_idx),
+        "p50_ms": float(np.percentile(L, 50)), "p99_ms": float(np.percentile(L, 99)),
+        "mean_ms": float(L.mean()), "submit_window_s": submit_s, "total_s": total_s,
+        "throughput_req_per_s": len(done_idx) / total_s,
+        "spin_updates_per_s": spin_upd / total_s,
+        "offered_tflops_alg": float(work_all.sum()) / seconds / 1e12,
+        "served_tflops_alg": work_done / max(st.busy_ms * 1e-3, 1e-9) / 1e12,
+        "device_busy_s": st.busy_ms / 1e3, "chunks": int(st.chunks), "max_active": int(st.max_active_seen),
+    }
+    leg["offered_over_capacity"] = leg["offered_tflops_alg"] / max(leg["served_tflops_alg"], 1e-9)
+    log(f"serving: {len(done_idx)}/{n_req} done, p50 {leg['p50_ms']:.0f} ms, p99 {leg['p99_ms']:.0f} ms, "
+        f"offered/capacity {leg['offered_over_capacity']:.2f}")
+    return leg
+
+
+def leg_accuracy(sb, ctx, S, nthreads):
+    from oracle import oracle as O
+    N, R = 2000, 8
+    J = sb.generate(sb.GEN_KN, N, 1.0, seed=3)
+    prob = ctx.problem(J)
+    c0 = O.c0_auto(O.BSB, J)
+    x32, y32 = O.init_state("uniform", N, R, seed=3)
+    x, y = x32.astype(np.float64), y32.astype(np.float64)
+    snaps = [0, S // 4, S // 2, 3 * S // 4, S - 1]
+    err = {"fp32": [], "split": []}
+    k = 0
+    t0 = time.perf_counter()
+    for sn in snaps:
+        O.run_f64(O.BSB, J, x, y, k, sn, S, 1.0, 1.0, c0, 1.0, 2, nthreads)
+        k = sn
+        xs = x[:, :N].astype(np.float32)
+        F64 = O.field_f64(J, np.ascontiguousarray(np.pad(xs.astype(np.float64), ((0, 0), (0, x.shape[1] - N)))))[:, :N]
+        for prec in ("fp32", "split"):
+            F = ctx.field(prob, xs, precision=prec, variant="bsb").astype(np.float64)
+            d = np.abs(F - F64)
+            err[prec].append({"step": sn, "max_abs": float(d.max()), "rel_max": float(d.max() / np.abs(F64).max()),
+                              "rel_rms": float(np.sqrt((d ** 2).mean() / (F64 ** 2).mean()))})
+    O.run_f64(O.BSB, J, x, y, k, S, S, 1.0, 1.0, c0, 1.0, 2, nthreads)
+    oracle_s = time.perf_counter() - t0
+    s64 = np.where(x[:, :N] > 0, 1, -1).astype(np.int8)
+    H64 = O.energy_i64(J, s64)
+    out = {"workload": f"K2000 +-1 seed 3, bSB, {R} replicas, {S} steps, dt=1.0; FP64 oracle on {nthreads} threads",
+           "field_error": err, "fp64_mean_energy": float(H64.mean()), "fp64_best_energy": float(H64.min()),
+           "oracle_s": oracle_s}
+    for prec in ("fp32", "split"):
+        sol = ctx.solve(prob, variant="bsb", precision=prec, steps=S, replicas=R, init="uniform", seed=3, dt=1.0,
+                        c0=c0)
+        out[f"{prec}_mean_energy"] = float(sol.energy.mean())
+        out[f"{prec}_best_energy"] = float(sol.energy.min())
+        out[f"{prec}_mean_gap_vs_fp64"] = (float(sol.energy.mean()) - out["fp64_mean_energy"]) / abs(out["fp64_mean_energy"])
+        out[f"{prec}_best_gap_vs_fp64"] = (float(sol.energy.min()) - out["fp64_best_energy"]) / abs(out["fp64_best_energy"])
+    log(f"accuracy: split mean gap {out['split_mean_gap_vs_fp64']:.2e}, "
+        f"field rel_max split {max(e['rel_max'] for e in err['split']):.2e}")
+    prob.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +256,15 @@ def main() -> int:
     ap.add_argument("--sb-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-form leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy leg")
+    ap.add_argument("--only-headline", action="store_true", help="skip every extra leg")
+    ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=1500)
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window (60 for the full config)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-drain-cap", type=float, default=120.0, help="max seconds to drain after arrivals")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -70,6 +282,8 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(replicas shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.only_headline:
+        args.no_cpu = args.no_precision = args.no_batched = args.no_serving = args.no_accuracy = True
 
     from annealing_amd import sb
 
@@ -78,8 +292,7 @@ def main() -> int:
     J = sb.generate(sb.GEN_SK, N, 1.0, seed=2)
     prob = ctx.problem(J)
     kw = dict(variant="bsb", precision="fp32", steps=S, replicas=R, init="uniform", seed=2, dt=1.0)
-    print(f"# bench: SK N={N} R={R} steps={S} fp32 bSB, warmup={args.warmup} timed={args.steps}",
-          file=sys.stderr, flush=True)
+    log(f"bench: SK N={N} R={R} steps={S} fp32 bSB, warmup={args.warmup} timed={args.steps}")
     wall, last = solve_timed(ctx, prob, args.steps, args.warmup, **kw)
     value = N * R * S * args.steps / wall
     avg_launch_s = last.step_ms / 1e3 / S
@@ -103,37 +316,24 @@ def main() -> int:
                    "N": N, "replicas": R, "sb_steps": S, "variant": "bsb", "precision": "fp32"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "fused step (FP32 MFMA 32x32x2)", "avg_launch_us": avg_launch_s * 1e6,
-                     "flops_per_launch": flops},
+                     "kernel": "k_step_fp32 (FP32 MFMA 32x32x2, fused bSB update)",
+                     "avg_launch_us": avg_launch_s * 1e6, "flops_per_launch": flops},
         "best_energy": float(last.energy.min()),
         "mean_energy": float(last.energy.mean()),
     }
-    print(f"# fp32 leg: {value:.4g} spin-updates/s, {avg_launch_s*1e6:.1f} us/launch", file=sys.stderr, flush=True)
+    log(f"fp32 leg: {value:.4g} spin-updates/s, {avg_launch_s*1e6:.1f} us/launch, {achieved:.1f} TFLOP/s")
 
     if not args.no_precision:
-        # K2000 +-1, R=4096: FP32 vs split (BF16 hi/lo, FP8 J) at the same seeds.
-        N3, R3 = 2000, 4096
-        J3 = sb.generate(sb.GEN_KN, N3, 1.0, seed=3)
-        p3 = ctx.problem(J3)
-        leg = {}
-        for name, kw3 in (("fp32_bsb", dict(variant="bsb", precision="fp32")),
-                          ("split_bsb", dict(variant="bsb", precision="split")),
-                          ("split_dsb", dict(variant="dsb", precision="split")),
-                          ("fp32_dsb", dict(variant="dsb", precision="fp32"))):
-            common = dict(steps=S, replicas=R3, init="uniform", seed=3, dt=1.0)
-            w3, r3 = solve_timed(ctx, p3, 1, 1, **kw3, **common)
-            leg[name] = {"spin_updates_per_s": N3 * R3 * S / w3, "solve_s": w3,
-                         "avg_launch_us": r3.step_ms * 1e3 / S,
-                         "mean_energy": float(r3.energy.mean()), "best_energy": float(r3.energy.min())}
-            print(f"# {name}: {leg[name]['spin_updates_per_s']:.4g} spin-updates/s", file=sys.stderr, flush=True)
-        leg["speedup_split_bsb_vs_fp32"] = leg["split_bsb"]["spin_updates_per_s"] / leg["fp32_bsb"]["spin_updates_per_s"]
-        leg["speedup_split_dsb_vs_fp32"] = leg["split_dsb"]["spin_updates_per_s"] / leg["fp32_dsb"]["spin_updates_per_s"]
-        leg["mean_energy_gap_split_vs_fp32_bsb"] = (
-            (leg["split_bsb"]["mean_energy"] - leg["fp32_bsb"]["mean_energy"]) / abs(leg["fp32_bsb"]["mean_energy"]))
-        leg["workload"] = "K2000 +-1 (FP8-exact J), R=4096, 2000 steps, dt=1.0, seed 3"
-        out["precision_form"] = leg
-        p3.close()
-
+        out["precision_form"] = leg_precision(sb, ctx, S)
+    if not args.no_batched:
+        out["batched"] = leg_batched(sb, ctx, args.batch_instances, args.batch_steps)
+    if not args.no_serving:
+        out["serving"] = leg_serving(sb, ctx, args.serve_seconds, args.serve_rate, args.serve_drain_cap)
+        out["p50_ms"] = out["serving"]["p50_ms"]
+        out["p99_ms"] = out["serving"]["p99_ms"]
+    threads = min(16, os.cpu_count() or 1)
+    if not args.no_accuracy:
+        out["accuracy"] = leg_accuracy(sb, ctx, S, threads)
     if not args.no_cpu:
         from oracle import oracle as O
         Rc, Sc = 2, S
@@ -145,6 +345,7 @@ def main() -> int:
         out["cpu_baseline"] = {"value": N * Rc * Sc / tc, "unit": "spin-updates/s", "cores": 1, "kind": "port",
                                "sample": f"same SK N={N} instance, {Rc} replicas x {Sc} steps, FP32 sequential-fmaf "
                                          f"oracle, 1 thread, {tc:.1f} s on {cpu_model()} (nproc={os.cpu_count()})"}
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.4g} spin-updates/s")
     prob.close()
     ctx.close()
     print(json.dumps(out))

@@ -202,6 +202,14 @@ SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, int32_t jfmt,
                    int32_t N, int32_t ldj, const float *dX, float *dXn, float *dY,
                    int32_t R, int32_t ldx, const sb_step_coeffs *c, void *stream);
 
+/* Local field of given amplitudes through the product kernels: F = J.X
+ * (variant SB_BSB/SB_ASB) or J.sign(X) (SB_DSB), host X and F [R][N], with the
+ * arithmetic of `precision` (FP32 MFMA, BF16 split, ordered).  The
+ * reference's field alone (cublasSgemv, m2_opt.cu:169); used to report field
+ * errors against an FP64 trajectory. */
+SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision, int32_t variant, const float *X,
+                    int32_t R, float *F);
+
 /* Initial state on device buffers ([R][ldx], padding zeroed). */
 SB_API int sb_init_state(sb_ctx *ctx, int32_t init, uint64_t seed, int32_t N, int32_t R,
                          int32_t ldx, float *dX, float *dY, void *stream);

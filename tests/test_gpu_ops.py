@@ -155,3 +155,25 @@ def test_sb_step_rejects_bad_layouts():
         assert lib.sb_step(ctx.handle, 0, p, 0, 60, 60, p, p, p, 1, 64, C.byref(c), None) == 1
         assert lib.sb_step(ctx.handle, 0, p, 0, 70, 64, p, p, p, 1, 64, C.byref(c), None) == 1
         assert lib.sb_step(ctx.handle, 0, p, 2, 60, 64, p, p, p, 1, 64, C.byref(c), None) == 1
+
+
+def test_sb_field_paths(orc):
+    N, R = 257, 6
+    J = sb.generate(sb.GEN_KN, N, 1.0, 12)
+    x, _ = orc.init_state("uniform", N, R, seed=3)
+    xs = x[:, :N].copy()
+    F32o = orc.field_f32(J, x)[:, :N]
+    F64 = orc.field_f64(J, x.astype(np.float64))[:, :N]
+    bound = N * 2.0**-24 * (np.abs(xs).astype(np.float64) @ np.abs(J).T.astype(np.float64))
+    with sb.Context(0) as ctx:
+        prob = ctx.problem(J)
+        assert np.array_equal(ctx.field(prob, xs, "ordered", "bsb"), F32o)
+        assert np.all(np.abs(ctx.field(prob, xs, "fp32", "bsb") - F64) <= 2 * bound)
+        assert np.all(np.abs(ctx.field(prob, xs, "split", "bsb") - F64) <= 2 * bound + 2.0**-16 * bound / (N * 2.0**-24))
+        exact = (np.where(xs > 0, 1, -1).astype(np.int64) @ J.astype(np.int64).T).astype(np.float32)
+        for prec in ("fp32", "split", "ordered"):
+            assert np.array_equal(ctx.field(prob, xs, prec, "dsb"), exact), prec
+        # aSB amplitudes beyond the wall are kept (no clamp in the field)
+        big = 3.0 * xs
+        assert np.allclose(ctx.field(prob, big, "fp32", "asb"), 3.0 * F64, rtol=1e-5, atol=1e-4)
+        prob.close()

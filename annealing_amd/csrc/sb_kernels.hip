@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace sbk {
 
 typedef float v16f __attribute__((ext_vector_type(16)));
@@ -133,33 +135,100 @@ __device__ __forceinline__ void write_planes(int kind, const v4f &xv, void *B, s
     }
 }
 
-// Epilogue of one 32x32 accumulator block.  MFMA 32x32 output layout: lane l,
-// register v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32, so each lane
-// owns four runs of four consecutive spins of one replica: one float4 each.
-__device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &acc, int row_base,
-                                               int col_base, int lane)
+// Plane entry of one spin (same encodings as write_planes).
+__device__ __forceinline__ void write_plane1(int kind, float x, void *B, size_t bplane, size_t o)
 {
-    int r = col_base + (lane & 31);
-    if (r >= a.R) return;
-    const float *xr = a.X + (size_t)r * a.ldx;
-    float *xnr = a.Xn + (size_t)r * a.ldx;
-    float *yr = a.Y + (size_t)r * a.ldx;
+    if (kind == BK_HILO) {
+        unsigned short *p = (unsigned short *)B;
+        const unsigned short hb = f2bf(x);
+        p[o] = hb;
+        p[bplane + o] = f2bf(x - bf2f(hb));
+    } else if (kind == BK_SIGN_FP8) {
+        ((unsigned char *)B)[o] = x > 0.0f ? (unsigned char)0x38 : (unsigned char)0xB8;
+    } else if (kind == BK_SIGN_BF16) {
+        ((unsigned short *)B)[o] = x > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
+    }
+}
+
+// Epilogue of one 32x32 accumulator block.  The tiles multiply with the state
+// fragment in the MFMA's A operand and the coupling fragment in its B operand,
+// so the block is F^T: lane l, register v holds replica
+// rep_base + 8*(v/4) + 4*(l/32) + v%4 and spin spin_base + l%32.  Every load
+// and store of a half-wave then covers 32 consecutive spins (128 bytes of X or
+// Y) of one replica: whole cache lines, not 16-byte pieces of 32 lines.
+__device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &acc, int spin_base, int rep_base,
+                                               int lane)
+{
+    const int i = spin_base + (lane & 31);
+    if (i >= a.N) return;
+    const int r0 = rep_base + 4 * (lane >> 5);
+    const size_t o0 = (size_t)r0 * a.ldx + i;
+    auto off = [&](int v) { return o0 + (size_t)(8 * (v >> 2) + (v & 3)) * a.ldx; };
+    if (rep_base + 32 <= a.R) {
+        // Whole block: all loads issue before the first use.
+        float xv[16], yv[16];
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        int i = row_base + 8 * g + 4 * (lane >> 5);
-        if (i >= a.N) continue;  // i < N implies i + 3 < ldx
-        v4f xv = *(const v4f *)(xr + i);
-        v4f yv = *(const v4f *)(yr + i);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            float xe = xv[e], ye = yv[e];
-            sb_update(a.c, acc[4 * g + e], xe, ye);
-            xv[e] = xe;
-            yv[e] = ye;
+        for (int v = 0; v < 16; ++v) {
+            xv[v] = a.X[off(v)];
+            yv[v] = a.Y[off(v)];
         }
-        *(v4f *)(xnr + i) = xv;
-        *(v4f *)(yr + i) = yv;
-        if (a.Bn) write_planes(a.bkind, xv, a.Bn, a.bplane, (size_t)r * a.ldx + i);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            sb_update(a.c, acc[v], xv[v], yv[v]);
+            a.Xn[off(v)] = xv[v];
+            a.Y[off(v)] = yv[v];
+        }
+        if (a.Bn) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) write_plane1(a.bkind, xv[v], a.Bn, a.bplane, off(v));
+        }
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        if (r0 + 8 * (v >> 2) + (v & 3) >= a.R) continue;
+        float x = a.X[off(v)], y = a.Y[off(v)];
+        sb_update(a.c, acc[v], x, y);
+        a.Xn[off(v)] = x;
+        a.Y[off(v)] = y;
+        if (a.Bn) write_plane1(a.bkind, x, a.Bn, a.bplane, off(v));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K pipeline shared by the families.  Two register stages hold chunks in
+// flight from global memory and LDS is double-buffered: chunk c+2's loads are
+// issued before chunk c is multiplied and stored to LDS after chunk c+1 is, so
+// every load has two chunks of MFMA work to arrive.  Stage selection is a
+// compile-time constant (the stages stay in registers).
+
+template <int S, typename T>
+__device__ __forceinline__ T &pick(T &s0, T &s1)
+{
+    if constexpr (S == 0) return s0;
+    else return s1;
+}
+
+template <typename Load, typename Store, typename Compute>
+__device__ __forceinline__ void kpipeline(int nch, Load gload, Store sstore, Compute compute)
+{
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    gload(I0{}, 0);
+    sstore(I0{}, 0);
+    if (nch > 1) gload(I1{}, 1);
+    __syncthreads();
+    for (int c = 0; c < nch; c += 2) {
+        if (c + 2 < nch) gload(I0{}, c + 2);
+        compute(0);
+        if (c + 1 < nch) sstore(I1{}, 1);
+        __syncthreads();
+        if (c + 1 < nch) {
+            if (c + 3 < nch) gload(I1{}, c + 3);
+            compute(1);
+            if (c + 2 < nch) sstore(I0{}, 0);
+            __syncthreads();
+        }
     }
 }
 
@@ -168,7 +237,10 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
 // double buffering.  Each lane reads a float4 of 4 consecutive k for its row
 // (A) and its replica (B); the four components feed four MFMAs, lane half
 // h = l/32 covering k = 8kk + 4h + c.  A and B use the same k assignment, so
-// the contraction is exact regardless of the hardware's k numbering.
+// the contraction is exact regardless of the hardware's k numbering.  In every
+// family the staged tiles are A = J rows and B = replica states, and the MFMA
+// takes the B fragment as its first operand: each accumulator block is F^T,
+// laid out for coalesced state I/O in epilogue_block.
 
 template <int WM, int WN, int BM, int BN>
 struct Fp32Tile {
@@ -194,8 +266,13 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
     const bool sgn = a.c.variant == SB_DSB;
     const int Kp = round_up_dev(a.N, KC);
 
-    v4f ra[T::LA], rb[T::LB];
-    auto gload = [&](int k0) {
+    struct Stage {
+        v4f a[T::LA], b[T::LB];
+    };
+    Stage st0, st1;
+    auto gload = [&](auto S, int ch) {
+        Stage &R = pick<decltype(S)::value>(st0, st1);
+        const int k0 = ch * KC;
 #pragma unroll
         for (int u = 0; u < T::LA; ++u) {
             int q = tid + u * NTHR;
@@ -203,7 +280,7 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
             int gi = row0 + rr;
             v4f v = {0.f, 0.f, 0.f, 0.f};
             if (q < MT * KC / 4 && gi < a.N) v = *(const v4f *)(J + (size_t)gi * a.ldj + k0 + 4 * c4);
-            ra[u] = v;
+            R.a[u] = v;
         }
 #pragma unroll
         for (int u = 0; u < T::LB; ++u) {
@@ -215,16 +292,17 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
                 v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c4);
                 if (sgn) v = (v4f){sgn1(v[0]), sgn1(v[1]), sgn1(v[2]), sgn1(v[3])};
             }
-            rb[u] = v;
+            R.b[u] = v;
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](auto S, int buf) {
+        Stage &R = pick<decltype(S)::value>(st0, st1);
 #pragma unroll
         for (int u = 0; u < T::LA; ++u) {
             int q = tid + u * NTHR;
             if (q < MT * KC / 4) {
                 int rr = q / (KC / 4), c4 = q % (KC / 4);
-                *(v4f *)(As + (buf * MT + rr) * LD + 4 * c4) = ra[u];
+                *(v4f *)(As + (buf * MT + rr) * LD + 4 * c4) = R.a[u];
             }
         }
 #pragma unroll
@@ -232,7 +310,7 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
             int q = tid + u * NTHR;
             if (q < NT * KC / 4) {
                 int rr = q / (KC / 4), c4 = q % (KC / 4);
-                *(v4f *)(Bs + (buf * NT + rr) * LD + 4 * c4) = rb[u];
+                *(v4f *)(Bs + (buf * NT + rr) * LD + 4 * c4) = R.b[u];
             }
         }
     };
@@ -245,15 +323,9 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[bm][bn][v] = 0.0f;
 
-    gload(0);
-    sstore(0);
-    __syncthreads();
     const int arow = wm * 32 * BM + (lane & 31), bcol = wn * 32 * BN + (lane & 31);
     const int koff = 4 * (lane >> 5);
-    int buf = 0;
-    for (int k0 = 0; k0 < Kp; k0 += KC) {
-        const bool more = k0 + KC < Kp;
-        if (more) gload(k0 + KC);
+    auto compute = [&](int buf) {
         const float *Ab = As + buf * MT * LD;
         const float *Bb = Bs + buf * NT * LD;
 #pragma unroll
@@ -269,12 +341,10 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
                 for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
                     for (int bn = 0; bn < BN; ++bn)
-                        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[bm][c], fb[bn][c], acc[bm][bn], 0, 0, 0);
+                        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(fb[bn][c], fa[bm][c], acc[bm][bn], 0, 0, 0);
         }
-        if (more) sstore(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
-    }
+    };
+    kpipeline((Kp + KC - 1) / KC, gload, sstore, compute);
 #pragma unroll
     for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
@@ -305,7 +375,7 @@ struct SplitTile {
     static constexpr int KC = 64, LDS_LD = KC + 8;  // in bf16 elements
 };
 
-template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES>
+template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES, bool SGN>
 __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned short *smem)
 {
     using T = SplitTile<WM, WN, BM, BN>;
@@ -318,6 +388,7 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     constexpr int B_PER = PLANES ? 8 : 4;
     constexpr int B_LOADS = NT * KC / B_PER;
     constexpr int LB = (B_LOADS + NTHR - 1) / NTHR;
+    constexpr int KK = KC / 16;
     unsigned short *As = smem;                          // [2][MT][LD]
     unsigned short *Bh = smem + 2 * MT * LD;            // [2][NT][LD]
     unsigned short *Bl = Bh + 2 * NT * LD;              // [2][NT][LD]
@@ -326,51 +397,56 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     const int row0 = tm * MT, col0 = tn * NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    const bool sgn = a.c.variant == SB_DSB;
     const int Kp = round_up_dev(a.N, KC);
     const unsigned short *Ph = (const unsigned short *)a.B;
     const unsigned short *Pl = Ph + a.bplane;
+    const u32x4 z4 = {0u, 0u, 0u, 0u};
 
-    u32x4 ra[LA];
-    v4f rb[PLANES ? 1 : LB];
-    u32x4 rbh[PLANES ? LB : 1], rbl[PLANES ? LB : 1];
-    auto gload = [&](int k0) {
+    struct Stage {
+        u32x4 ra[LA];
+        v4f rb[PLANES ? 1 : LB];
+        u32x4 rbh[PLANES ? LB : 1], rbl[PLANES ? LB : 1];
+    };
+    Stage st0, st1;
+    auto gload = [&](auto S, int ch) {
+        Stage &R = pick<decltype(S)::value>(st0, st1);
+        const int k0 = ch * KC;
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
             int q = tid + u * NTHR;
             int rr = q / (KC / A_PER), c = q % (KC / A_PER);
             int gi = row0 + rr;
-            u32x4 v = {0u, 0u, 0u, 0u};
+            u32x4 v = z4;
             if (q < A_LOADS && gi < a.N) {
                 if (FP8J)
                     v = *(const u32x4 *)((const unsigned char *)a.J + (size_t)gi * a.ldj + k0 + 16 * c);
                 else
                     v = *(const u32x4 *)((const unsigned short *)a.J + (size_t)gi * a.ldj + k0 + 8 * c);
             }
-            ra[u] = v;
+            R.ra[u] = v;
         }
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
             int rr = q / (KC / B_PER), c = q % (KC / B_PER);
             int gr = col0 + rr;
+            const bool ok = q < B_LOADS && gr < a.R;
             if (PLANES) {
-                u32x4 h = {0u, 0u, 0u, 0u}, l = {0u, 0u, 0u, 0u};
-                if (q < B_LOADS && gr < a.R) {
-                    size_t o = (size_t)gr * a.ldx + k0 + 8 * c;
-                    h = *(const u32x4 *)(Ph + o);
-                    if (!sgn) l = *(const u32x4 *)(Pl + o);
+                const size_t o = (size_t)gr * a.ldx + k0 + 8 * c;
+                R.rbh[u] = z4;
+                R.rbl[u] = z4;
+                if (ok) {
+                    R.rbh[u] = *(const u32x4 *)(Ph + o);
+                    if (!SGN) R.rbl[u] = *(const u32x4 *)(Pl + o);
                 }
-                rbh[u] = h;
-                rbl[u] = l;
             } else {
-                v4f v = {0.f, 0.f, 0.f, 0.f};
-                if (q < B_LOADS && gr < a.R) v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c);
-                rb[u] = v;
+                R.rb[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+                if (ok) R.rb[u] = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c);
             }
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](auto S, int buf) {
+        Stage &R = pick<decltype(S)::value>(st0, st1);
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
             int q = tid + u * NTHR;
@@ -380,11 +456,11 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
                 if (FP8J) {
                     unsigned int w[8];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) fp8x4_to_bf16x4(ra[u][e], w[2 * e], w[2 * e + 1]);
+                    for (int e = 0; e < 4; ++e) fp8x4_to_bf16x4(R.ra[u][e], w[2 * e], w[2 * e + 1]);
                     *(u32x4 *)dst = (u32x4){w[0], w[1], w[2], w[3]};
                     *(u32x4 *)(dst + 8) = (u32x4){w[4], w[5], w[6], w[7]};
                 } else {
-                    *(u32x4 *)dst = ra[u];
+                    *(u32x4 *)dst = R.ra[u];
                 }
             }
         }
@@ -394,14 +470,14 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
             if (q < B_LOADS) {
                 int rr = q / (KC / B_PER), c = q % (KC / B_PER);
                 if (PLANES) {
-                    *(u32x4 *)(Bh + (buf * NT + rr) * LD + 8 * c) = rbh[u];
-                    if (!sgn) *(u32x4 *)(Bl + (buf * NT + rr) * LD + 8 * c) = rbl[u];
+                    *(u32x4 *)(Bh + (buf * NT + rr) * LD + 8 * c) = R.rbh[u];
+                    if (!SGN) *(u32x4 *)(Bl + (buf * NT + rr) * LD + 8 * c) = R.rbl[u];
                 } else {
                     u16x4 h, l;
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        float v = rb[u][e];
-                        if (sgn) {
+                        float v = R.rb[u][e];
+                        if (SGN) {
                             h[e] = v > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
                             l[e] = 0;
                         } else {
@@ -411,7 +487,7 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
                         }
                     }
                     *(u16x4 *)(Bh + (buf * NT + rr) * LD + 4 * c) = h;
-                    if (!sgn) *(u16x4 *)(Bl + (buf * NT + rr) * LD + 4 * c) = l;
+                    if (!SGN) *(u16x4 *)(Bl + (buf * NT + rr) * LD + 4 * c) = l;
                 }
             }
         }
@@ -425,46 +501,48 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[bm][bn][v] = 0.0f;
 
-    gload(0);
-    sstore(0);
-    __syncthreads();
     const int arow = wm * 32 * BM + (lane & 31), bcol = wn * 32 * BN + (lane & 31);
     const int koff = 8 * (lane >> 5);
-    int buf = 0;
-    for (int k0 = 0; k0 < Kp; k0 += KC) {
-        const bool more = k0 + KC < Kp;
-        if (more) gload(k0 + KC);
+    struct Frag {
+        bf16x8 a[BM], h[BN], l[BN];
+    };
+    auto compute = [&](int buf) {
         const unsigned short *Ab = As + buf * MT * LD;
         const unsigned short *Bhb = Bh + buf * NT * LD;
         const unsigned short *Blb = Bl + buf * NT * LD;
-#pragma unroll
-        for (int kk = 0; kk < KC / 16; ++kk) {
-            bf16x8 fa[BM], fh[BN], fl[BN];
+        auto ldfrag = [&](Frag &F, int kk) {
 #pragma unroll
             for (int bm = 0; bm < BM; ++bm)
-                fa[bm] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ab + (arow + 32 * bm) * LD + 16 * kk + koff));
+                F.a[bm] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ab + (arow + 32 * bm) * LD + 16 * kk + koff));
 #pragma unroll
             for (int bn = 0; bn < BN; ++bn) {
-                fh[bn] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Bhb + (bcol + 32 * bn) * LD + 16 * kk + koff));
-                if (!sgn) fl[bn] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Blb + (bcol + 32 * bn) * LD + 16 * kk + koff));
+                F.h[bn] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Bhb + (bcol + 32 * bn) * LD + 16 * kk + koff));
+                if (!SGN)
+                    F.l[bn] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Blb + (bcol + 32 * bn) * LD + 16 * kk + koff));
             }
+        };
+        auto mma = [&](const Frag &F) {
 #pragma unroll
             for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
                 for (int bn = 0; bn < BN; ++bn)
-                    acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[bm], fh[bn], acc[bm][bn], 0, 0, 0);
-            if (!sgn) {
+                    acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F.h[bn], F.a[bm], acc[bm][bn], 0, 0, 0);
+            if (!SGN) {
 #pragma unroll
                 for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
                     for (int bn = 0; bn < BN; ++bn)
-                        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[bm], fl[bn], acc[bm][bn], 0, 0, 0);
+                        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F.l[bn], F.a[bm], acc[bm][bn], 0, 0, 0);
             }
+        };
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) {
+            Frag f;
+            ldfrag(f, kk);
+            mma(f);
         }
-        if (more) sstore(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
-    }
+    };
+    kpipeline((Kp + KC - 1) / KC, gload, sstore, compute);
 #pragma unroll
     for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
@@ -507,10 +585,15 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
     const int Kp = round_up_dev(a.N, 64);
     const unsigned char *P = (const unsigned char *)a.B;
 
-    u32x4 ra[LA];
-    unsigned int rb[PLANES ? 1 : LB];
-    u32x4 rbp[PLANES ? LB : 1];
-    auto gload = [&](int k0) {
+    struct Stage {
+        u32x4 ra[LA];
+        unsigned int rb[PLANES ? 1 : LB];
+        u32x4 rbp[PLANES ? LB : 1];
+    };
+    Stage st0, st1;
+    auto gload = [&](auto S, int ch) {
+        Stage &R = pick<decltype(S)::value>(st0, st1);
+        const int k0 = ch * KC;
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
             int q = tid + u * NTHR;
@@ -519,7 +602,7 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
             u32x4 v = {0u, 0u, 0u, 0u};
             if (q < A_LOADS && gi < a.N && gk < Kp)
                 v = *(const u32x4 *)((const unsigned char *)a.J + (size_t)gi * a.ldj + gk);
-            ra[u] = v;
+            R.ra[u] = v;
         }
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
@@ -529,7 +612,7 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
             if (PLANES) {
                 u32x4 v = {0u, 0u, 0u, 0u};
                 if (q < B_LOADS && gr < a.R && gk < Kp) v = *(const u32x4 *)(P + (size_t)gr * a.ldx + gk);
-                rbp[u] = v;
+                R.rbp[u] = v;
             } else {
                 unsigned int s4 = 0u;  // FP8 zero bytes beyond N: J is zero there anyway
                 if (q < B_LOADS && gr < a.R && gk < Kp) {
@@ -538,17 +621,18 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 #pragma unroll
                     for (int e = 0; e < 4; ++e) s4 |= (v[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
                 }
-                rb[u] = s4;
+                R.rb[u] = s4;
             }
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](auto S, int buf) {
+        Stage &R = pick<decltype(S)::value>(st0, st1);
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
             int q = tid + u * NTHR;
             if (q < A_LOADS) {
                 int rr = q / (KC / 16), c = q % (KC / 16);
-                *(u32x4 *)(As + (buf * MT + rr) * LD + 16 * c) = ra[u];
+                *(u32x4 *)(As + (buf * MT + rr) * LD + 16 * c) = R.ra[u];
             }
         }
 #pragma unroll
@@ -557,9 +641,9 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
             if (q < B_LOADS) {
                 int rr = q / (KC / B_PER), c = q % (KC / B_PER);
                 if (PLANES)
-                    *(u32x4 *)(Bs + (buf * NT + rr) * LD + 16 * c) = rbp[u];
+                    *(u32x4 *)(Bs + (buf * NT + rr) * LD + 16 * c) = R.rbp[u];
                 else
-                    *(unsigned int *)(Bs + (buf * NT + rr) * LD + 4 * c) = rb[u];
+                    *(unsigned int *)(Bs + (buf * NT + rr) * LD + 4 * c) = R.rb[u];
             }
         }
     };
@@ -572,15 +656,9 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[bm][bn][v] = 0.0f;
 
-    gload(0);
-    sstore(0);
-    __syncthreads();
     const int arow = wm * 32 * BM + (lane & 31), bcol = wn * 32 * BN + (lane & 31);
     const int koff = 32 * (lane >> 5);
-    int buf = 0;
-    for (int k0 = 0; k0 < Kp; k0 += KC) {
-        const bool more = k0 + KC < Kp;
-        if (more) gload(k0 + KC);
+    auto compute = [&](int buf) {
         const unsigned char *Ab = As + buf * MT * LD;
         const unsigned char *Bb = Bs + buf * NT * LD;
 #pragma unroll
@@ -605,12 +683,10 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 #pragma unroll
                 for (int bn = 0; bn < BN; ++bn)
                     acc[bm][bn] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                        fa[bm], fb[bn], acc[bm][bn], 0, 0, 0, 127, 0, 127);
+                        fb[bn], fa[bm], acc[bm][bn], 0, 0, 0, 127, 0, 127);
         }
-        if (more) sstore(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
-    }
+    };
+    kpipeline((Kp + KC - 1) / KC, gload, sstore, compute);
 #pragma unroll
     for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
@@ -629,12 +705,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_step_fp32(StepArgs a)
     fp32_tile<WM, WN, BM, BN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
-template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES>
+template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES, bool SGN>
 __global__ __launch_bounds__(64 * WM * WN) void k_step_split(StepArgs a)
 {
     using T = SplitTile<WM, WN, BM, BN>;
     __shared__ __attribute__((aligned(16))) unsigned short smem[2 * (T::MT + 2 * T::NT) * T::LDS_LD];
-    split_tile<WM, WN, BM, BN, FP8J, PLANES>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+    split_tile<WM, WN, BM, BN, FP8J, PLANES, SGN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
 }
 
 template <int WM, int WN, int BM, int BN, bool PLANES>
@@ -671,11 +747,11 @@ __global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
 // Shape tables (4-wave and 8-wave workgroups).
 
 #define SB_FP32_SHAPES(X) X(2, 2, 1, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1) \
-    X(2, 4, 1, 1) X(4, 2, 1, 1) X(4, 2, 1, 2) X(2, 4, 2, 1) X(4, 2, 2, 1)
+    X(2, 4, 1, 1) X(4, 2, 1, 1) X(4, 2, 1, 2) X(2, 4, 2, 1) X(4, 2, 2, 1) X(2, 2, 4, 1) X(2, 2, 2, 4)
 #define SB_SPLIT_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
-    X(4, 2, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1)
+    X(4, 2, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1)
 #define SB_FP8_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
-    X(4, 2, 2, 2) X(2, 4, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1)
+    X(4, 2, 2, 2) X(2, 4, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1)
 
 #define SB_SHAPE_ENTRY(wm, wn, bm, bn) TileShape{wm, wn, bm, bn},
 static const TileShape kFp32Shapes[] = {SB_FP32_SHAPES(SB_SHAPE_ENTRY)};
@@ -717,19 +793,26 @@ int rows_per_group(int tiles_m, int tiles_n, int mt, int nt, int bytes_j, int by
 typedef void (*StepKernel)(StepArgs);
 
 #define SB_FP32_PTR(wm, wn, bm, bn) &k_step_fp32<wm, wn, bm, bn>,
-#define SB_SPLIT_BF16_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, false>,
-#define SB_SPLIT_FP8_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, false>,
 #define SB_DSB_FP8_PTR(wm, wn, bm, bn) &k_step_dsb_fp8<wm, wn, bm, bn, false>,
-#define SB_SPLIT_BF16_PL_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, true>,
-#define SB_SPLIT_FP8_PL_PTR(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true>,
 #define SB_DSB_FP8_PL_PTR(wm, wn, bm, bn) &k_step_dsb_fp8<wm, wn, bm, bn, true>,
 static const StepKernel kFp32Kernels[] = {SB_FP32_SHAPES(SB_FP32_PTR)};
-static const StepKernel kSplitBf16Kernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_BF16_PTR)};
-static const StepKernel kSplitFp8Kernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_FP8_PTR)};
 static const StepKernel kDsbFp8Kernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PTR)};
-static const StepKernel kSplitBf16PlKernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_BF16_PL_PTR)};
-static const StepKernel kSplitFp8PlKernels[] = {SB_SPLIT_SHAPES(SB_SPLIT_FP8_PL_PTR)};
 static const StepKernel kDsbFp8PlKernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PL_PTR)};
+// Split kernels indexed [FP8 J][planes][sign B][shape].
+#define SB_SPLIT_PTR(F, P, G) \
+    {SB_SPLIT_SHAPES(SB_SPLIT_PTR_1##F##P##G)}
+#define SB_SPLIT_PTR_1000(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, false, false>,
+#define SB_SPLIT_PTR_1001(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, false, true>,
+#define SB_SPLIT_PTR_1010(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, true, false>,
+#define SB_SPLIT_PTR_1011(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, true, true>,
+#define SB_SPLIT_PTR_1100(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, false, false>,
+#define SB_SPLIT_PTR_1101(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, false, true>,
+#define SB_SPLIT_PTR_1110(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true, false>,
+#define SB_SPLIT_PTR_1111(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true, true>,
+#define SB_COUNT_ONE(wm, wn, bm, bn) +1
+static const StepKernel kSplitKernels[2][2][2][0 SB_SPLIT_SHAPES(SB_COUNT_ONE)] = {
+    {{SB_SPLIT_PTR(0, 0, 0), SB_SPLIT_PTR(0, 0, 1)}, {SB_SPLIT_PTR(0, 1, 0), SB_SPLIT_PTR(0, 1, 1)}},
+    {{SB_SPLIT_PTR(1, 0, 0), SB_SPLIT_PTR(1, 0, 1)}, {SB_SPLIT_PTR(1, 1, 0), SB_SPLIT_PTR(1, 1, 1)}}};
 
 hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStream_t s)
 {
@@ -749,8 +832,10 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
     StepKernel k = nullptr;
     switch (family) {
     case FAM_FP32: k = kFp32Kernels[shape_idx]; break;
-    case FAM_SPLIT_BF16: k = pl ? kSplitBf16PlKernels[shape_idx] : kSplitBf16Kernels[shape_idx]; break;
-    case FAM_SPLIT_FP8: k = pl ? kSplitFp8PlKernels[shape_idx] : kSplitFp8Kernels[shape_idx]; break;
+    case FAM_SPLIT_BF16:
+    case FAM_SPLIT_FP8:
+        k = kSplitKernels[family == FAM_SPLIT_FP8][pl][a.c.variant == SB_DSB][shape_idx];
+        break;
     case FAM_DSB_FP8: k = pl ? kDsbFp8PlKernels[shape_idx] : kDsbFp8Kernels[shape_idx]; break;
     default: return hipErrorInvalidValue;
     }
@@ -817,7 +902,7 @@ int batch_shape_solo_index(int family, int idx)
 
 size_t batch_lds_bytes() { return kBatchLds; }
 
-__global__ __launch_bounds__(256) void k_batch_step(const BatchInstance *__restrict__ insts,
+__global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__restrict__ insts,
                                                     const BatchItem *__restrict__ items,
                                                     const int *__restrict__ wg_begin, int chunk_step)
 {
@@ -846,22 +931,30 @@ __global__ __launch_bounds__(256) void k_batch_step(const BatchInstance *__restr
         a.tiles_n = I.tiles_n;
         a.rpg = 0;
         a.c = step_coeffs(I.variant, k, I.steps, I.dt, I.a0, I.c0, I.K, I.M);
-        const int sel = I.family * 8 + I.bshape;
-        // One case per (family, batch shape); must follow the SB_BATCH_* lists.
+        const int sel = (I.family * 2 + (I.variant == SB_DSB)) * 8 + I.bshape;
+        unsigned short *sm16 = (unsigned short *)smem_dyn;
+        // One case per (family, sign B, batch shape); must follow the SB_BATCH_* lists.
         switch (sel) {
-        case FAM_FP32 * 8 + 0: fp32_tile<2, 2, 1, 1>(a, item.tile, (float *)smem_dyn); break;
-        case FAM_FP32 * 8 + 1: fp32_tile<2, 2, 2, 1>(a, item.tile, (float *)smem_dyn); break;
-        case FAM_FP32 * 8 + 2: fp32_tile<4, 1, 1, 1>(a, item.tile, (float *)smem_dyn); break;
-        case FAM_FP32 * 8 + 3: fp32_tile<1, 4, 1, 1>(a, item.tile, (float *)smem_dyn); break;
-        case FAM_SPLIT_BF16 * 8 + 0: split_tile<2, 2, 1, 1, false, true>(a, item.tile, (unsigned short *)smem_dyn); break;
-        case FAM_SPLIT_BF16 * 8 + 1: split_tile<2, 2, 2, 1, false, true>(a, item.tile, (unsigned short *)smem_dyn); break;
-        case FAM_SPLIT_BF16 * 8 + 2: split_tile<4, 1, 1, 1, false, true>(a, item.tile, (unsigned short *)smem_dyn); break;
-        case FAM_SPLIT_FP8 * 8 + 0: split_tile<2, 2, 1, 1, true, true>(a, item.tile, (unsigned short *)smem_dyn); break;
-        case FAM_SPLIT_FP8 * 8 + 1: split_tile<2, 2, 2, 1, true, true>(a, item.tile, (unsigned short *)smem_dyn); break;
-        case FAM_SPLIT_FP8 * 8 + 2: split_tile<4, 1, 1, 1, true, true>(a, item.tile, (unsigned short *)smem_dyn); break;
-        case FAM_DSB_FP8 * 8 + 0: dsb_fp8_tile<2, 2, 1, 1, true>(a, item.tile, smem_dyn); break;
-        case FAM_DSB_FP8 * 8 + 1: dsb_fp8_tile<2, 2, 2, 1, true>(a, item.tile, smem_dyn); break;
-        case FAM_DSB_FP8 * 8 + 2: dsb_fp8_tile<4, 1, 1, 1, true>(a, item.tile, smem_dyn); break;
+        case (FAM_FP32 * 2 + 0) * 8 + 0:
+        case (FAM_FP32 * 2 + 1) * 8 + 0: fp32_tile<2, 2, 1, 1>(a, item.tile, (float *)smem_dyn); break;
+        case (FAM_FP32 * 2 + 0) * 8 + 1:
+        case (FAM_FP32 * 2 + 1) * 8 + 1: fp32_tile<2, 2, 2, 1>(a, item.tile, (float *)smem_dyn); break;
+        case (FAM_FP32 * 2 + 0) * 8 + 2:
+        case (FAM_FP32 * 2 + 1) * 8 + 2: fp32_tile<4, 1, 1, 1>(a, item.tile, (float *)smem_dyn); break;
+        case (FAM_FP32 * 2 + 0) * 8 + 3:
+        case (FAM_FP32 * 2 + 1) * 8 + 3: fp32_tile<1, 4, 1, 1>(a, item.tile, (float *)smem_dyn); break;
+        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, false, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, false, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, false, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 1) * 8 + 0: split_tile<2, 2, 1, 1, false, true, true>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 1) * 8 + 1: split_tile<2, 2, 2, 1, false, true, true>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 1) * 8 + 2: split_tile<4, 1, 1, 1, false, true, true>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, true, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, true, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, true, true, false>(a, item.tile, sm16); break;
+        case (FAM_DSB_FP8 * 2 + 1) * 8 + 0: dsb_fp8_tile<2, 2, 1, 1, true>(a, item.tile, smem_dyn); break;
+        case (FAM_DSB_FP8 * 2 + 1) * 8 + 1: dsb_fp8_tile<2, 2, 2, 1, true>(a, item.tile, smem_dyn); break;
+        case (FAM_DSB_FP8 * 2 + 1) * 8 + 2: dsb_fp8_tile<4, 1, 1, 1, true>(a, item.tile, smem_dyn); break;
         default: break;
         }
         __syncthreads();  // LDS is reused by the next item

@@ -11,7 +11,7 @@ from annealing_amd import sb
 pytestmark = pytest.mark.gpu
 
 # Compiled shapes per family (csrc/sb_kernels.hip: SB_*_SHAPES, SB_BATCH_*).
-SHAPES = {0: 11, 1: 8, 2: 8, 3: 9}
+SHAPES = {0: 13, 1: 10, 2: 10, 3: 11}
 BATCH_SHAPES = {0: 4, 1: 3, 2: 3, 3: 3}
 N, R, STEPS = 300, 40, 30
 

@@ -562,8 +562,18 @@ int check_params(const sb_params *p, int N, double sigma, double *c0_out)
     return SB_OK;
 }
 
-std::vector<int> candidate_families(int precision, int variant, const sb_problem *prob)
+std::vector<int> candidate_families(int precision, int variant, const sb_problem *prob, int R)
 {
+    // Few replicas: the matrix-vector families (their field does not depend on
+    // which of them runs, see the thin kernels).
+    if (R <= kThinMaxR) {
+        if (precision == SB_PREC_FP32) return {FAM_THIN_F32};
+        if (precision == SB_PREC_SPLIT) {
+            if (prob->Jfp8) return {FAM_THIN_FP8, FAM_THIN_BF16};
+            return {FAM_THIN_BF16};
+        }
+        return {};
+    }
     if (precision == SB_PREC_FP32) return {FAM_FP32};
     if (precision == SB_PREC_SPLIT) {
         if (prob->Jfp8) {
@@ -578,10 +588,13 @@ std::vector<int> candidate_families(int precision, int variant, const sb_problem
 const void *family_J(int family, const sb_problem *prob)
 {
     switch (family) {
-    case FAM_FP32: return prob->J32;
-    case FAM_SPLIT_BF16: return prob->Jbf16;
+    case FAM_FP32:
+    case FAM_THIN_F32: return prob->J32;
+    case FAM_SPLIT_BF16:
+    case FAM_THIN_BF16: return prob->Jbf16;
     case FAM_SPLIT_FP8:
-    case FAM_DSB_FP8: return prob->Jfp8;
+    case FAM_DSB_FP8:
+    case FAM_THIN_FP8: return prob->Jfp8;
     default: return nullptr;
     }
 }
@@ -681,7 +694,7 @@ static int tune_candidates(sb_ctx *ctx, const sb_problem *prob, int variant, int
 int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int variant, int R, int *family,
                   int *shape)
 {
-    std::vector<int> fams = candidate_families(precision, variant, prob);
+    std::vector<int> fams = candidate_families(precision, variant, prob, R);
     if (fams.empty()) return SB_E_INVALID;
     const int N = prob->N;
     // Debug override SB_SHAPE=<family>:<shape> (used by the shape-sweep tests).
@@ -882,12 +895,13 @@ extern "C" SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, in
     if (ldj < N || ldx < N || ldj % kSpinPad || ldx % kSpinPad) return SB_E_INVALID;
     if (c->variant < SB_ASB || c->variant > SB_DSB) return SB_E_INVALID;
     int family;
+    const bool thin = R <= kThinMaxR;
     if (precision == SB_PREC_FP32 || precision == SB_PREC_ORDERED) {
         if (jfmt != SB_J_FP32) return SB_E_INVALID;
-        family = FAM_FP32;
+        family = thin ? FAM_THIN_F32 : FAM_FP32;
     } else if (precision == SB_PREC_SPLIT) {
-        if (jfmt == SB_J_BF16) family = FAM_SPLIT_BF16;
-        else if (jfmt == SB_J_FP8) family = c->variant == SB_DSB ? FAM_DSB_FP8 : FAM_SPLIT_FP8;
+        if (jfmt == SB_J_BF16) family = thin ? FAM_THIN_BF16 : FAM_SPLIT_BF16;
+        else if (jfmt == SB_J_FP8) family = thin ? FAM_THIN_FP8 : c->variant == SB_DSB ? FAM_DSB_FP8 : FAM_SPLIT_FP8;
         else return SB_E_INVALID;
     } else {
         return SB_E_INVALID;

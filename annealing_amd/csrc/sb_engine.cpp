@@ -15,8 +15,14 @@ using namespace sbk;
 
 namespace sbh {
 
-int engine_family(int precision, int variant, const sb_problem *prob)
+int engine_family(int precision, int variant, const sb_problem *prob, int R)
 {
+    if (R <= kThinMaxR) {
+        // Same family set as a solo solve; FP8 and BF16 thin fields are equal.
+        if (precision == SB_PREC_FP32) return FAM_THIN_F32;
+        if (precision == SB_PREC_SPLIT) return prob->Jfp8 ? FAM_THIN_FP8 : FAM_THIN_BF16;
+        return -1;
+    }
     if (precision == SB_PREC_FP32) return FAM_FP32;
     if (precision == SB_PREC_SPLIT) {
         // Batched instances stream J from HBM every step: the FP8 form when
@@ -56,7 +62,7 @@ size_t Engine::state_bytes(int N, int R)
 int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
 {
     const int N = prob->N, ld = prob->ld, R = p.replicas;
-    const int family = engine_family(p.precision, p.variant, prob);
+    const int family = engine_family(p.precision, p.variant, prob, R);
     if (family < 0) {
         set_error("batched solve supports SB_PREC_FP32 and SB_PREC_SPLIT");
         return SB_E_INVALID;
@@ -229,7 +235,9 @@ int Engine::rebuild()
         I.c0 = s.c0;
         I.K = s.params.K;
         if (!I.active) continue;
-        const double tile_cost = (double)sh.mt() * sh.nt() * (double)round_up(N, kSpinPad);
+        // Cost model: MFMA tiles by their area; a thin tile streams its rows of
+        // J on the VALU at roughly the cost of a 64-replica MFMA row block.
+        const double tile_cost = (double)sh.mt() * (sh.rows ? 64 : sh.nt()) * (double)round_up(N, kSpinPad);
         for (int t = 0; t < I.tiles_m * I.tiles_n; ++t) per_slot[i].push_back({tile_cost, (int)i, t});
         slot_cost[i] = tile_cost * I.tiles_m * I.tiles_n;
         total_items += per_slot[i].size();

@@ -81,7 +81,7 @@ class Engine {
 };
 
 // Kernel family for a slot (batched kernels only; not SB_PREC_ORDERED).
-int engine_family(int precision, int variant, const sb_problem *prob);
+int engine_family(int precision, int variant, const sb_problem *prob, int R);
 // Autotuned batch shape of a family for (N, R), cached per context.
 int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int variant, int R, int *bshape);
 

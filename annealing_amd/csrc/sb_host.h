@@ -56,7 +56,7 @@ int check_params(const sb_params *p, int N, double sigma, double *c0_out);
 // Pick the kernel family for a precision/variant/problem (before tuning).
 // Returns the list of families whose results are identical for this case;
 // the autotuner may choose any of them.
-std::vector<int> candidate_families(int precision, int variant, const sb_problem *prob);
+std::vector<int> candidate_families(int precision, int variant, const sb_problem *prob, int R);
 
 const void *family_J(int family, const sb_problem *prob);
 

@@ -67,10 +67,16 @@ inline size_t plane_bytes(int kind)
 // Tile shapes.  A workgroup has WM x WN waves; each wave owns BM x BN blocks
 // of 32 x 32 (spins x replicas).  Every shape accumulates each F_i in the same
 // k order, so the choice never changes results.
+// Replica counts up to this run on the thin (matrix-vector) families.
+constexpr int kThinMaxR = 4;
+
+// MFMA tile: wm x wn waves, each bm x bn blocks of 32x32.  Thin tile (rows >
+// 0): wm waves, each owning `rows` spins for all R <= kThinMaxR replicas.
 struct TileShape {
     int wm, wn, bm, bn;
-    int mt() const { return 32 * bm * wm; }
-    int nt() const { return 32 * bn * wn; }
+    int rows;
+    int mt() const { return rows ? wm * rows : 32 * bm * wm; }
+    int nt() const { return rows ? kThinMaxR : 32 * bn * wn; }
     int threads() const { return 64 * wm * wn; }
 };
 
@@ -80,8 +86,12 @@ enum Family {
     FAM_SPLIT_BF16 = 1, // BF16 J x BF16 hi/lo X (or BF16 sign X for dSB)
     FAM_SPLIT_FP8 = 2,  // FP8 J widened to BF16 x BF16 hi/lo X
     FAM_DSB_FP8 = 3,    // FP8 J x FP8 sign(X) on the f8f6f4 MFMA (dSB only)
-    FAM_COUNT = 4
+    FAM_THIN_F32 = 4,   // R <= kThinMaxR: FP32 J streamed, FP32 FMA on the VALU
+    FAM_THIN_BF16 = 5,  // same with BF16 J
+    FAM_THIN_FP8 = 6,   // same with FP8 J
+    FAM_COUNT = 7
 };
+inline bool is_thin(int family) { return family >= FAM_THIN_F32 && family <= FAM_THIN_FP8; }
 
 // Number of compiled tile shapes per family and their table.
 int shape_count(int family);

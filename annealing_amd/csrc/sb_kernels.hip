@@ -695,6 +695,118 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 }
 
 // ---------------------------------------------------------------------------
+// Thin families (R <= kThinMaxR).  With a handful of replicas the field is a
+// matrix-vector product that streams J once from HBM; MFMA tiles would be at
+// least 7/8 padding, so it runs as FP32 fmaf on the VALU.  Wave w of tile t
+// owns spins [(4t + w) * ROWS, +ROWS).  Lane l accumulates, per row and
+// replica, the 16 consecutive k at 16 * (l + 64 * it) in ascending k with
+// fmaf, over it = 0, 1, ...; the 64 lane sums are then combined by an xor
+// butterfly, which leaves the same sum in every lane (each level adds two
+// values that are already equal across the pair).  The order depends neither
+// on ROWS nor on how J is stored, so every thin shape, and the FP32, BF16 and
+// FP8 forms of the same J values, give the same field.
+
+template <int JB>
+__device__ __forceinline__ void thin_load_j(const void *J, size_t o, float (&v)[16])
+{
+    if constexpr (JB == 4) {
+        const v4f *p = (const v4f *)((const float *)J + o);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const v4f t = p[g];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[4 * g + e] = t[e];
+        }
+    } else if constexpr (JB == 2) {
+        const u32x4 *p = (const u32x4 *)((const unsigned short *)J + o);
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            const u32x4 t = p[g];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[8 * g + 2 * e] = __builtin_bit_cast(float, t[e] << 16);
+                v[8 * g + 2 * e + 1] = __builtin_bit_cast(float, t[e] & 0xffff0000u);
+            }
+        }
+    } else {
+        const u32x4 t = *(const u32x4 *)((const unsigned char *)J + o);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            unsigned int lo2, hi2;  // BF16 pairs of bytes 0,1 and 2,3
+            fp8x4_to_bf16x4(t[e], lo2, hi2);
+            v[4 * e + 0] = __builtin_bit_cast(float, lo2 << 16);
+            v[4 * e + 1] = __builtin_bit_cast(float, lo2 & 0xffff0000u);
+            v[4 * e + 2] = __builtin_bit_cast(float, hi2 << 16);
+            v[4 * e + 3] = __builtin_bit_cast(float, hi2 & 0xffff0000u);
+        }
+    }
+}
+
+template <int ROWS, int JB, bool SGN>
+__device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = (tile * 4 + wave) * ROWS;
+    if (row0 >= a.N) return;
+    float acc[ROWS][kThinMaxR];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q)
+#pragma unroll
+        for (int r = 0; r < kThinMaxR; ++r) acc[q][r] = 0.0f;
+    for (int k0 = 16 * lane; k0 < a.N; k0 += 1024) {
+        // k0 + 16 <= round_up(N, 16) <= ld: the group is in bounds.
+        float jv[ROWS][16];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            if (row0 + q < a.N) {
+                thin_load_j<JB>(a.J, (size_t)(row0 + q) * a.ldj + k0, jv[q]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) jv[q][e] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kThinMaxR; ++r) {
+            if (r >= a.R) break;
+            const v4f *xp = (const v4f *)(a.X + (size_t)r * a.ldx + k0);
+            float xv[16];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const v4f t = xp[g];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) xv[4 * g + e] = SGN ? sgn1(t[e]) : t[e];
+            }
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[q][r] = __builtin_fmaf(jv[q][e], xv[e], acc[q][r]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q)
+#pragma unroll
+        for (int r = 0; r < kThinMaxR; ++r)
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) acc[q][r] += __shfl_xor(acc[q][r], m);
+    // Lane q * kThinMaxR + r updates spin row0 + q of replica r.
+    if (lane >= ROWS * kThinMaxR) return;
+    const int q = lane / kThinMaxR, r = lane % kThinMaxR, i = row0 + q;
+    if (r >= a.R || i >= a.N) return;
+    float F = 0.0f;
+#pragma unroll
+    for (int qq = 0; qq < ROWS; ++qq)
+#pragma unroll
+        for (int rr = 0; rr < kThinMaxR; ++rr)
+            if (qq == q && rr == r) F = acc[qq][rr];
+    const size_t o = (size_t)r * a.ldx + i;
+    float x = a.X[o], y = a.Y[o];
+    sb_update(a.c, F, x, y);
+    a.Xn[o] = x;
+    a.Y[o] = y;
+    if (a.Bn) write_plane1(a.bkind, x, a.Bn, a.bplane, o);
+}
+
+// ---------------------------------------------------------------------------
 // Kernel entry points per family and shape.
 
 template <int WM, int WN, int BM, int BN>
@@ -703,6 +815,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_step_fp32(StepArgs a)
     using T = Fp32Tile<WM, WN, BM, BN>;
     __shared__ __attribute__((aligned(16))) float smem[2 * (T::MT + T::NT) * T::LDS_LD];
     fp32_tile<WM, WN, BM, BN>(a, xcd_remap(blockIdx.x, gridDim.x), smem);
+}
+
+template <int ROWS, int JB, bool SGN>
+__global__ __launch_bounds__(256) void k_step_thin(StepArgs a)
+{
+    thin_tile<ROWS, JB, SGN>(a, blockIdx.x);
 }
 
 template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES, bool SGN>
@@ -753,10 +871,15 @@ __global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
 #define SB_FP8_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
     X(4, 2, 2, 2) X(2, 4, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1)
 
-#define SB_SHAPE_ENTRY(wm, wn, bm, bn) TileShape{wm, wn, bm, bn},
+// Thin shapes: spins per wave (4 waves per workgroup).
+#define SB_THIN_SHAPES(X) X(1) X(2) X(4) X(8)
+
+#define SB_SHAPE_ENTRY(wm, wn, bm, bn) TileShape{wm, wn, bm, bn, 0},
+#define SB_THIN_ENTRY(rows) TileShape{4, 1, 1, 1, rows},
 static const TileShape kFp32Shapes[] = {SB_FP32_SHAPES(SB_SHAPE_ENTRY)};
 static const TileShape kSplitShapes[] = {SB_SPLIT_SHAPES(SB_SHAPE_ENTRY)};
 static const TileShape kFp8Shapes[] = {SB_FP8_SHAPES(SB_SHAPE_ENTRY)};
+static const TileShape kThinShapes[] = {SB_THIN_SHAPES(SB_THIN_ENTRY)};
 #undef SB_SHAPE_ENTRY
 
 int shape_count(int family)
@@ -766,6 +889,9 @@ int shape_count(int family)
     case FAM_SPLIT_BF16:
     case FAM_SPLIT_FP8: return (int)(sizeof(kSplitShapes) / sizeof(kSplitShapes[0]));
     case FAM_DSB_FP8: return (int)(sizeof(kFp8Shapes) / sizeof(kFp8Shapes[0]));
+    case FAM_THIN_F32:
+    case FAM_THIN_BF16:
+    case FAM_THIN_FP8: return (int)(sizeof(kThinShapes) / sizeof(kThinShapes[0]));
     default: return 0;
     }
 }
@@ -776,6 +902,9 @@ TileShape shape_at(int family, int idx)
     case FAM_FP32: return kFp32Shapes[idx];
     case FAM_SPLIT_BF16:
     case FAM_SPLIT_FP8: return kSplitShapes[idx];
+    case FAM_THIN_F32:
+    case FAM_THIN_BF16:
+    case FAM_THIN_FP8: return kThinShapes[idx];
     default: return kFp8Shapes[idx];
     }
 }
@@ -810,15 +939,37 @@ static const StepKernel kDsbFp8PlKernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PL_PTR)};
 #define SB_SPLIT_PTR_1110(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true, false>,
 #define SB_SPLIT_PTR_1111(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true, true>,
 #define SB_COUNT_ONE(wm, wn, bm, bn) +1
+#define SB_COUNT_ONE1(rows) +1
 static const StepKernel kSplitKernels[2][2][2][0 SB_SPLIT_SHAPES(SB_COUNT_ONE)] = {
     {{SB_SPLIT_PTR(0, 0, 0), SB_SPLIT_PTR(0, 0, 1)}, {SB_SPLIT_PTR(0, 1, 0), SB_SPLIT_PTR(0, 1, 1)}},
     {{SB_SPLIT_PTR(1, 0, 0), SB_SPLIT_PTR(1, 0, 1)}, {SB_SPLIT_PTR(1, 1, 0), SB_SPLIT_PTR(1, 1, 1)}}};
+
+// Thin kernels indexed [J format: FP32, BF16, FP8][sign x][shape].
+#define SB_THIN_PTR_F32(rows) &k_step_thin<rows, 4, false>,
+#define SB_THIN_PTR_F32S(rows) &k_step_thin<rows, 4, true>,
+#define SB_THIN_PTR_BF16(rows) &k_step_thin<rows, 2, false>,
+#define SB_THIN_PTR_BF16S(rows) &k_step_thin<rows, 2, true>,
+#define SB_THIN_PTR_FP8(rows) &k_step_thin<rows, 1, false>,
+#define SB_THIN_PTR_FP8S(rows) &k_step_thin<rows, 1, true>,
+static const StepKernel kThinKernels[3][2][0 SB_THIN_SHAPES(SB_COUNT_ONE1)] = {
+    {{SB_THIN_SHAPES(SB_THIN_PTR_F32)}, {SB_THIN_SHAPES(SB_THIN_PTR_F32S)}},
+    {{SB_THIN_SHAPES(SB_THIN_PTR_BF16)}, {SB_THIN_SHAPES(SB_THIN_PTR_BF16S)}},
+    {{SB_THIN_SHAPES(SB_THIN_PTR_FP8)}, {SB_THIN_SHAPES(SB_THIN_PTR_FP8S)}}};
 
 hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStream_t s)
 {
     if (shape_idx < 0 || shape_idx >= shape_count(family)) return hipErrorInvalidValue;
     TileShape sh = shape_at(family, shape_idx);
     StepArgs a = a_in;
+    if (is_thin(family)) {
+        if (a.R > kThinMaxR) return hipErrorInvalidValue;
+        a.tiles_m = (a.N + sh.mt() - 1) / sh.mt();
+        a.tiles_n = 1;
+        a.rpg = 1;
+        StepKernel k = kThinKernels[family - FAM_THIN_F32][a.c.variant == SB_DSB][shape_idx];
+        hipLaunchKernelGGL(k, dim3(a.tiles_m), dim3(sh.threads()), 0, s, a);
+        return hipGetLastError();
+    }
     a.tiles_m = (a.N + sh.mt() - 1) / sh.mt();
     a.tiles_n = (a.R + sh.nt() - 1) / sh.nt();
     int grid = a.tiles_m * a.tiles_n;
@@ -865,6 +1016,7 @@ hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
 static const TileShape kBatchFp32[] = {SB_BATCH_FP32(SB_SHAPE_ENTRY)};
 static const TileShape kBatchSplit[] = {SB_BATCH_SPLIT(SB_SHAPE_ENTRY)};
 static const TileShape kBatchFp8[] = {SB_BATCH_FP8(SB_SHAPE_ENTRY)};
+static const TileShape kBatchThin[] = {SB_THIN_ENTRY(1) SB_THIN_ENTRY(2) SB_THIN_ENTRY(4)};
 #undef SB_SHAPE_ENTRY
 
 constexpr size_t kBatchLds = 73728;  // max over the shapes above (split 128x64)
@@ -876,6 +1028,9 @@ int batch_shape_count(int family)
     case FAM_SPLIT_BF16:
     case FAM_SPLIT_FP8: return (int)(sizeof(kBatchSplit) / sizeof(kBatchSplit[0]));
     case FAM_DSB_FP8: return (int)(sizeof(kBatchFp8) / sizeof(kBatchFp8[0]));
+    case FAM_THIN_F32:
+    case FAM_THIN_BF16:
+    case FAM_THIN_FP8: return (int)(sizeof(kBatchThin) / sizeof(kBatchThin[0]));
     default: return 0;
     }
 }
@@ -886,6 +1041,9 @@ TileShape batch_shape_at(int family, int idx)
     case FAM_FP32: return kBatchFp32[idx];
     case FAM_SPLIT_BF16:
     case FAM_SPLIT_FP8: return kBatchSplit[idx];
+    case FAM_THIN_F32:
+    case FAM_THIN_BF16:
+    case FAM_THIN_FP8: return kBatchThin[idx];
     default: return kBatchFp8[idx];
     }
 }
@@ -895,7 +1053,7 @@ int batch_shape_solo_index(int family, int idx)
     TileShape b = batch_shape_at(family, idx);
     for (int s = 0; s < shape_count(family); ++s) {
         TileShape t = shape_at(family, s);
-        if (t.wm == b.wm && t.wn == b.wn && t.bm == b.bm && t.bn == b.bn) return s;
+        if (t.wm == b.wm && t.wn == b.wn && t.bm == b.bm && t.bn == b.bn && t.rows == b.rows) return s;
     }
     return -1;
 }
@@ -955,6 +1113,17 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 0: dsb_fp8_tile<2, 2, 1, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 1: dsb_fp8_tile<2, 2, 2, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 2: dsb_fp8_tile<4, 1, 1, 1, true>(a, item.tile, smem_dyn); break;
+#define SB_THIN_CASES(fam, jb)                                                                  \
+        case (fam * 2 + 0) * 8 + 0: thin_tile<1, jb, false>(a, item.tile); break;              \
+        case (fam * 2 + 0) * 8 + 1: thin_tile<2, jb, false>(a, item.tile); break;              \
+        case (fam * 2 + 0) * 8 + 2: thin_tile<4, jb, false>(a, item.tile); break;              \
+        case (fam * 2 + 1) * 8 + 0: thin_tile<1, jb, true>(a, item.tile); break;               \
+        case (fam * 2 + 1) * 8 + 1: thin_tile<2, jb, true>(a, item.tile); break;               \
+        case (fam * 2 + 1) * 8 + 2: thin_tile<4, jb, true>(a, item.tile); break;
+        SB_THIN_CASES(FAM_THIN_F32, 4)
+        SB_THIN_CASES(FAM_THIN_BF16, 2)
+        SB_THIN_CASES(FAM_THIN_FP8, 1)
+#undef SB_THIN_CASES
         default: break;
         }
         __syncthreads();  // LDS is reused by the next item

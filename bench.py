@@ -96,6 +96,32 @@ def leg_precision(sb, ctx, S):
     return leg
 
 
+def leg_matvec(sb, ctx, S):
+    """R = 1: the reference's own shape (one replica, one SGEMV per step).  The
+    thin kernels stream J once per step; achieved GB/s counts J bytes only."""
+    leg = {"workload": f"R=1 bSB, {S} steps, dt=1.0; SK (FP32 J, split form reads BF16 J) and K_N +-1 "
+                       "(split form reads FP8 J) generated on the device, seed 2"}
+    for N, kind, kname in ((16384, sb.GEN_SK, "sk"), (16384, sb.GEN_KN, "kn"), (2000, sb.GEN_SK, "sk")):
+        prob = ctx.generate(kind, N, 1.0, seed=2)
+        for prec in ("fp32", "split"):
+            if kname == "sk" and prec == "split" and N != 16384:
+                continue
+            if kname == "kn" and prec == "fp32":
+                continue
+            jb = 4 if prec == "fp32" else (1 if prob.split_jfmt == sb.J_FP8 else 2)
+            w, r = solve_timed(ctx, prob, 1, 1, variant="bsb", precision=prec, steps=S, replicas=1,
+                               init="uniform", seed=2, dt=1.0)
+            us = r.step_ms * 1e3 / S
+            gbs = N * N * jb / (us * 1e-6) / 1e9
+            name = f"{kname}{N}_{prec}_r1"
+            leg[name] = {"avg_launch_us": us, "j_bytes_per_launch": N * N * jb, "j_format": {4: "fp32", 2: "bf16",
+                         1: "fp8"}[jb], "achieved_gbs": gbs, "frac_of_hbm_peak": gbs / PEAK_HBM_GBS,
+                         "spin_updates_per_s": N * S / w}
+            log(f"matvec {name}: {us:.1f} us/launch, {gbs:.0f} GB/s of J")
+        prob.close()
+    return leg
+
+
 def leg_batched(sb, ctx, n_inst, steps):
     rng = np.random.default_rng(4)
     Ns = rng.integers(64, 4097, n_inst)
@@ -259,6 +285,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy leg")
+    ap.add_argument("--no-matvec", action="store_true", help="skip the R=1 matrix-vector leg")
     ap.add_argument("--only-headline", action="store_true", help="skip every extra leg")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=1500)
@@ -284,6 +311,7 @@ def main() -> int:
         return 2
     if args.only_headline:
         args.no_cpu = args.no_precision = args.no_batched = args.no_serving = args.no_accuracy = True
+        args.no_matvec = True
 
     from annealing_amd import sb
 
@@ -325,6 +353,8 @@ def main() -> int:
 
     if not args.no_precision:
         out["precision_form"] = leg_precision(sb, ctx, S)
+    if not args.no_matvec:
+        out["matvec"] = leg_matvec(sb, ctx, 200)
     if not args.no_batched:
         out["batched"] = leg_batched(sb, ctx, args.batch_instances, args.batch_steps)
     if not args.no_serving:
@@ -336,7 +366,7 @@ def main() -> int:
         out["accuracy"] = leg_accuracy(sb, ctx, S, threads)
     if not args.no_cpu:
         from oracle import oracle as O
-        Rc, Sc = 2, S
+        Rc, Sc = 16, S
         x, y = O.init_state("uniform", N, Rc, seed=2)
         c0 = O.c0_auto(O.BSB, J)
         t0 = time.perf_counter()

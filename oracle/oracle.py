@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 ASB, BSB, DSB = 0, 1, 2
+ORDER_SEQ, ORDER_LANES = 0, 1  # field summation orders (sb_oracle.h)
 
 _lib = None
 
@@ -35,10 +36,12 @@ def load():
             "orc_coeffs": (None, [I, I, I, D, D, D, P]),
             "orc_field_f32": (None, [I, I, P, I, P, I, I, P]),
             "orc_field_f64": (None, [I, I, P, I, P, I, I, P]),
+            "orc_field_f32_lanes": (None, [I, I, P, I, P, I, I, P]),
             "orc_bsb_update_f32": (None, [F, F, F, F, C.POINTER(F), C.POINTER(F)]),
             "orc_asb_local_f32": (None, [C.POINTER(F), C.POINTER(F), F, D, D, D, I]),
             "orc_run_f32": (I, [I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
             "orc_run_f64": (I, [I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
+            "orc_run_f32_order": (I, [I, I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
             "orc_spins_f32": (None, [I, I, P, I, P]),
             "orc_energy_i64": (I, [I, I, P, I, P, P]),
             "orc_energy_f64": (None, [I, I, P, I, P, P]),
@@ -105,13 +108,14 @@ def coeffs(variant, k, steps, dt, a0, c0):
     return out
 
 
-def field_f32(J, x, use_sign=False):
+def field_f32(J, x, use_sign=False, order=ORDER_SEQ):
     J = np.ascontiguousarray(J, np.float32)
     x = np.ascontiguousarray(x, np.float32)
     N = J.shape[0]
     R, ld = x.shape
     F = np.zeros((R, ld), np.float32)
-    load().orc_field_f32(N, R, _p(J), J.shape[1], _p(x), ld, int(use_sign), _p(F))
+    fn = load().orc_field_f32 if order == ORDER_SEQ else load().orc_field_f32_lanes
+    fn(N, R, _p(J), J.shape[1], _p(x), ld, int(use_sign), _p(F))
     return F
 
 
@@ -137,13 +141,14 @@ def asb_local_f32(x, y, p, a0, K, dts, M):
     return xx.value, yy.value
 
 
-def run_f32(variant, J, x, y, k0, k1, steps, dt=1.0, a0=1.0, c0=0.0, K=1.0, M=2, nthreads=1):
+def run_f32(variant, J, x, y, k0, k1, steps, dt=1.0, a0=1.0, c0=0.0, K=1.0, M=2, nthreads=1,
+            order=ORDER_SEQ):
     """Advance float32 state (x, y) [R, ld] in place over steps [k0, k1)."""
     J = np.ascontiguousarray(J, np.float32)
     assert x.dtype == np.float32 and y.dtype == np.float32 and x.flags.c_contiguous
     R, ld = x.shape
-    rc = load().orc_run_f32(variant, J.shape[0], R, _p(J), J.shape[1], _p(x), _p(y), ld, k0, k1, steps,
-                            dt, a0, c0, K, M, nthreads)
+    rc = load().orc_run_f32_order(order, variant, J.shape[0], R, _p(J), J.shape[1], _p(x), _p(y), ld, k0,
+                                  k1, steps, dt, a0, c0, K, M, nthreads)
     if rc:
         raise RuntimeError(f"orc_run_f32 failed: {rc}")
 
@@ -209,11 +214,11 @@ def parse_edgelist(text, one_based=False):
 
 
 def solve_f32(variant, J, R, steps, init="uniform", seed=0, dt=1.0, a0=1.0, c0=None, K=1.0, M=2,
-              nthreads=1):
+              nthreads=1, order=ORDER_SEQ):
     """Whole solve on the CPU: init, (aSB prologue), steps; returns x, y [R, ld]."""
     N = J.shape[0]
     if c0 is None or c0 <= 0:
         c0 = c0_auto(variant, J, a0)
     x, y = init_state(init, N, R, seed)
-    run_f32(variant, J, x, y, 0, steps, steps, dt, a0, c0, K, M, nthreads)
+    run_f32(variant, J, x, y, 0, steps, steps, dt, a0, c0, K, M, nthreads, order)
     return x, y, c0

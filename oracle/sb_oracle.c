@@ -168,6 +168,49 @@ static void field_rows_f32(int N, const float *J, int ldj, const float *xs, floa
     }
 }
 
+/* The same per-row fmaf accumulation regrouped into 64 lane sums: lane l
+ * takes the groups of 16 consecutive k starting at 16 (l + 64 it), it = 0,
+ * 1, ..., each in ascending k, and the 64 sums are combined pairwise at lane
+ * distances 32, 16, 8, 4, 2, 1.  Positions k in [N, round_up(N, 16)) enter as
+ * fmaf(0, pad, acc) with pad = xs_pad[k] (the zero padding of the state, or
+ * its sign for dSB).  This is the summation order of the thin kernels; xs_pad
+ * holds round_up(N, 16) values. */
+static void field_rows_f32_lanes(int N, const float *J, int ldj, const float *xs_pad, float *F)
+{
+    for (int i = 0; i < N; ++i) {
+        const float *row = J + (size_t)i * ldj;
+        float p[64];
+        for (int l = 0; l < 64; ++l) {
+            float a = 0.f;
+            for (int k0 = 16 * l; k0 < N; k0 += 1024)
+                for (int e = 0; e < 16; ++e) {
+                    const int k = k0 + e;
+                    a = fmaf(k < N ? row[k] : 0.0f, xs_pad[k], a);
+                }
+            p[l] = a;
+        }
+        for (int m = 32; m >= 1; m >>= 1) {
+            float q[64];
+            for (int l = 0; l < 64; ++l) q[l] = p[l] + p[l ^ m];
+            memcpy(p, q, sizeof(p));
+        }
+        F[i] = p[0];
+    }
+}
+
+static void field_order(int order, int variant, int N, const float *J, int ldj, const float *x,
+                        float *s, float *F);
+
+void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, int ld,
+                         int use_sign, float *F)
+{
+    float *s = (float *)malloc(sizeof(float) * (size_t)(N + 16));
+    for (int r = 0; r < R; ++r)
+        field_order(ORC_ORDER_LANES, use_sign ? ORC_DSB : ORC_BSB, N, J, ldj, x + (size_t)r * ld, s,
+                    F + (size_t)r * ld);
+    free(s);
+}
+
 void orc_field_f32(int N, int R, const float *J, int ldj, const float *x, int ld,
                    int use_sign, float *F)
 {
@@ -273,7 +316,29 @@ static void asb_local_f64(double *x, double *y, double p, double a0, double K,
 
 /* ---- solver loops ------------------------------------------------------ */
 
-static int run_f32_replica(int variant, int N, const float *J, int ldj, float *x,
+/* Field of one replica in the chosen order; s has round_up(N, 16) entries. */
+static void field_order(int order, int variant, int N, const float *J, int ldj, const float *x,
+                        float *s, float *F)
+{
+    const int Kp = (N + 15) / 16 * 16;
+    const int sign = variant == ORC_DSB;
+    if (order == ORC_ORDER_SEQ) {
+        const float *src = x;
+        if (sign) {
+            for (int j = 0; j < N; ++j) s[j] = sgnf(x[j]);
+            src = s;
+        }
+        field_rows_f32(N, J, ldj, src, F);
+        return;
+    }
+    for (int j = 0; j < Kp; ++j) {
+        const float v = j < N ? x[j] : 0.0f;
+        s[j] = sign ? sgnf(v) : v;
+    }
+    field_rows_f32_lanes(N, J, ldj, s, F);
+}
+
+static int run_f32_replica(int order, int variant, int N, const float *J, int ldj, float *x,
                            float *y, float *F, float *s, int k0, int k1, int steps,
                            double dt, double a0, double c0, double K, int M)
 {
@@ -284,15 +349,10 @@ static int run_f32_replica(int variant, int N, const float *J, int ldj, float *x
         if (variant == ORC_ASB) {
             /* m2_opt.cu:168-169: local sub-steps, then the kick with new x */
             for (int i = 0; i < N; ++i) orc_asb_local_f32(&x[i], &y[i], cf[3], a0, K, dts, M);
-            field_rows_f32(N, J, ldj, x, F);
+            field_order(order, variant, N, J, ldj, x, s, F);
             for (int i = 0; i < N; ++i) y[i] = fmaf(cf[1], F[i], y[i]);
         } else {
-            const float *src = x;
-            if (variant == ORC_DSB) {
-                for (int j = 0; j < N; ++j) s[j] = sgnf(x[j]);
-                src = s;
-            }
-            field_rows_f32(N, J, ldj, src, F);
+            field_order(order, variant, N, J, ldj, x, s, F);
             for (int i = 0; i < N; ++i) orc_bsb_update_f32(F[i], cf[0], cf[1], cf[2], &x[i], &y[i]);
         }
     }
@@ -313,8 +373,17 @@ int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
                 float *y, int ld, int k0, int k1, int steps, double dt, double a0,
                 double c0, double K, int M, int nthreads)
 {
+    return orc_run_f32_order(ORC_ORDER_SEQ, variant, N, R, J, ldj, x, y, ld, k0, k1, steps, dt,
+                             a0, c0, K, M, nthreads);
+}
+
+int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int ldj, float *x,
+                      float *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                      double c0, double K, int M, int nthreads)
+{
     int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
     if (rc) return rc;
+    if (order != ORC_ORDER_SEQ && order != ORC_ORDER_LANES) return ORC_E_INVALID;
     if (!orc_has_fma()) return ORC_E_NOFMA;
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
@@ -322,12 +391,12 @@ int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
 #endif
     {
         float *F = (float *)malloc(sizeof(float) * (size_t)N);
-        float *s = (float *)malloc(sizeof(float) * (size_t)N);
+        float *s = (float *)malloc(sizeof(float) * (size_t)(N + 16));
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
         for (int r = 0; r < R; ++r)
-            run_f32_replica(variant, N, J, ldj, x + (size_t)r * ld, y + (size_t)r * ld,
+            run_f32_replica(order, variant, N, J, ldj, x + (size_t)r * ld, y + (size_t)r * ld,
                             F, s, k0, k1, steps, dt, a0, c0, K, M);
         free(F);
         free(s);

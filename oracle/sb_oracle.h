@@ -36,6 +36,10 @@ extern "C" {
 #endif
 
 enum { ORC_ASB = 0, ORC_BSB = 1, ORC_DSB = 2 };
+/* Field summation orders: SEQ is the plain j = 0..N-1 fmaf chain per row;
+ * LANES regroups it into 64 strided lane sums combined pairwise (the order of
+ * the few-replica matrix-vector kernels; see orc_field_f32_lanes). */
+enum { ORC_ORDER_SEQ = 0, ORC_ORDER_LANES = 1 };
 
 enum {
     ORC_OK = 0,
@@ -84,6 +88,13 @@ void orc_field_f32(int N, int R, const float *J, int ldj, const float *x, int ld
                    int use_sign, float *F);
 void orc_field_f64(int N, int R, const float *J, int ldj, const double *x, int ld,
                    int use_sign, double *F);
+/* The same products in the LANES order: per row, lane l = 0..63 accumulates
+ * with fmaf the groups of 16 consecutive k starting at 16 (l + 64 it),
+ * it = 0, 1, ..., in ascending k (positions in [N, round_up(N, 16)) add
+ * fmaf(0, pad, acc), pad = 0 or sign(0) = -1 for dSB); then the 64 sums are
+ * combined as p[l] = p[l] + p[l ^ m] for m = 32, 16, ..., 1 and p[0] taken. */
+void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, int ld,
+                         int use_sign, float *F);
 
 /* FP32 epilogues, exactly as the product defines them. */
 void orc_bsb_update_f32(float F, float alpha, float beta, float gamma,
@@ -98,6 +109,11 @@ void orc_asb_local_f32(float *x, float *y, float p, double a0, double K,
 int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
                 float *y, int ld, int k0, int k1, int steps, double dt, double a0,
                 double c0, double K, int M, int nthreads);
+
+/* orc_run_f32 with the field in the given order (ORC_ORDER_SEQ or _LANES). */
+int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int ldj,
+                      float *x, float *y, int ld, int k0, int k1, int steps, double dt,
+                      double a0, double c0, double K, int M, int nthreads);
 
 /* Same schedule in FP64 (the FP64 oracle trajectory). */
 int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,

@@ -172,6 +172,52 @@ def test_field_f32_error_bound(orc):
     assert np.all(np.abs(F32 - F64) <= bound)
 
 
+def test_lane_order_field_pins(orc):
+    # Integer couplings times integer-valued state are exact in any order, so
+    # the lane-order field must equal the integer product: a dropped, repeated
+    # or misplaced 16-group or a wrong butterfly step fails.  The sizes cover
+    # one partial group, whole lane sweeps and partial last sweeps.
+    for N in (5, 16, 70, 1023, 1025, 2100):
+        J = RNG.integers(-16, 17, (N, N)).astype(np.float32)
+        J = np.triu(J, 1)
+        J = J + J.T
+        ld = orc.pad_to(N)
+        xi = np.zeros((3, ld), np.int64)
+        xi[:, :N] = RNG.integers(-3, 4, (3, N))
+        x = xi.astype(np.float32)
+        exact = xi[:, :N] @ J.astype(np.int64).T
+        F = orc.field_f32(J, x, order=orc.ORDER_LANES)
+        assert np.array_equal(F[:, :N].astype(np.int64), exact), N
+        s = np.where(x[:, :N] > 0, 1, -1).astype(np.int64)
+        Fs = orc.field_f32(J, x, use_sign=True, order=orc.ORDER_LANES)
+        assert np.array_equal(Fs[:, :N].astype(np.int64), s @ J.astype(np.int64).T), N
+    # Real couplings: within the FP32 bound of the FP64 field, and a different
+    # rounding sequence from the sequential chain.
+    N = 1500
+    J = (RNG.standard_normal((N, N)) / np.sqrt(N)).astype(np.float32)
+    J = np.triu(J, 1)
+    J = J + J.T
+    x, _ = orc.init_state("uniform", N, 3, seed=4)
+    Fl = orc.field_f32(J, x, order=orc.ORDER_LANES)[:, :N]
+    Fq = orc.field_f32(J, x)[:, :N]
+    F64 = orc.field_f64(J, x.astype(np.float64))[:, :N]
+    bound = N * 2.0**-24 * (np.abs(x[:, :N]).astype(np.float64) @ np.abs(J).T.astype(np.float64))
+    assert np.all(np.abs(Fl - F64) <= bound)
+    assert np.any(Fl != Fq)
+
+
+def test_lane_order_run_equals_sequential_for_exact_fields(orc):
+    # dSB with integer J: every field is an exact integer, so the two orders
+    # give the same trajectory bit for bit.
+    N, R, steps = 300, 4, 60
+    J = RNG.integers(-2, 3, (N, N)).astype(np.float32)
+    J = np.triu(J, 1)
+    J = J + J.T
+    xa, ya, _ = orc.solve_f32(orc.DSB, J, R, steps, seed=5, dt=0.5)
+    xb, yb, _ = orc.solve_f32(orc.DSB, J, R, steps, seed=5, dt=0.5, order=orc.ORDER_LANES)
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+
+
 # ---- R7 energy (m2_opt.cu:192-202) ----------------------------------------------
 
 def _sym_int(N, lo, hi, rng):

@@ -209,27 +209,49 @@ __device__ __forceinline__ T &pick(T &s0, T &s1)
     else return s1;
 }
 
+// Range-checked tile loads.  A buffer resource over [base, base + bytes)
+// returns zeros for any load that starts past the end (the hardware's range
+// check), so rows past N and replicas past R need no branch: with the loads
+// unconditional the compiler can wait for the older of two chunks in flight
+// instead of draining every load before each LDS store.  Word 3 is the raw
+// 32-bit data format of the gfx9 family.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void *base, long long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ u32x4 rload16(__amdgpu_buffer_rsrc_t r, unsigned int off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+// An offset past any tile range (tiles stay below 2^31 bytes).
+constexpr unsigned int kOutOfRange = 0x80000000u;
+
 template <typename Load, typename Store, typename Compute>
 __device__ __forceinline__ void kpipeline(int nch, Load gload, Store sstore, Compute compute)
 {
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
+    // Loads and LDS stores are unconditional, also for the (at most two)
+    // chunks past the last: the tile loads are range-checked (in-range bytes
+    // or zeros, never a fault) and such a stage lands in an LDS buffer that no
+    // compute reads.  Every path then has the same loads in flight, so the
+    // wait before a store covers only that stage's loads.
     gload(I0{}, 0);
     sstore(I0{}, 0);
-    if (nch > 1) gload(I1{}, 1);
+    gload(I1{}, 1);
     __syncthreads();
-    for (int c = 0; c < nch; c += 2) {
-        if (c + 2 < nch) gload(I0{}, c + 2);
+    int c = 0;
+    for (; c + 1 < nch; c += 2) {
+        gload(I0{}, c + 2);
         compute(0);
-        if (c + 1 < nch) sstore(I1{}, 1);
+        sstore(I1{}, 1);
         __syncthreads();
-        if (c + 1 < nch) {
-            if (c + 3 < nch) gload(I1{}, c + 3);
-            compute(1);
-            if (c + 2 < nch) sstore(I0{}, 0);
-            __syncthreads();
-        }
+        gload(I1{}, c + 3);
+        compute(1);
+        sstore(I0{}, 0);
+        __syncthreads();
     }
+    if (c < nch) compute(0);  // odd chunk count: the last chunk is in buffer 0
 }
 
 // ---------------------------------------------------------------------------
@@ -266,8 +288,14 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
     const bool sgn = a.c.variant == SB_DSB;
     const int Kp = round_up_dev(a.N, KC);
 
+    // Rows row0.. of J and replicas col0.. of X; loads past N or R read zeros.
+    const __amdgpu_buffer_rsrc_t rJ =
+        tile_rsrc(J + (size_t)row0 * a.ldj, (long long)min(MT, a.N - row0) * a.ldj * 4);
+    const __amdgpu_buffer_rsrc_t rX =
+        tile_rsrc(a.X + (size_t)col0 * a.ldx, (long long)min(NT, a.R - col0) * a.ldx * 4);
+
     struct Stage {
-        v4f a[T::LA], b[T::LB];
+        u32x4 a[T::LA], b[T::LB];
     };
     Stage st0, st1;
     auto gload = [&](auto S, int ch) {
@@ -275,24 +303,13 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
         const int k0 = ch * KC;
 #pragma unroll
         for (int u = 0; u < T::LA; ++u) {
-            int q = tid + u * NTHR;
-            int rr = q / (KC / 4), c4 = q % (KC / 4);
-            int gi = row0 + rr;
-            v4f v = {0.f, 0.f, 0.f, 0.f};
-            if (q < MT * KC / 4 && gi < a.N) v = *(const v4f *)(J + (size_t)gi * a.ldj + k0 + 4 * c4);
-            R.a[u] = v;
+            const int q = tid + u * NTHR, rr = q / (KC / 4), c4 = q % (KC / 4);
+            R.a[u] = rload16(rJ, (unsigned)(rr * a.ldj + k0 + 4 * c4) * 4u);
         }
 #pragma unroll
         for (int u = 0; u < T::LB; ++u) {
-            int q = tid + u * NTHR;
-            int rr = q / (KC / 4), c4 = q % (KC / 4);
-            int gr = col0 + rr;
-            v4f v = {0.f, 0.f, 0.f, 0.f};
-            if (q < NT * KC / 4 && gr < a.R) {
-                v = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c4);
-                if (sgn) v = (v4f){sgn1(v[0]), sgn1(v[1]), sgn1(v[2]), sgn1(v[3])};
-            }
-            R.b[u] = v;
+            const int q = tid + u * NTHR, rr = q / (KC / 4), c4 = q % (KC / 4);
+            R.b[u] = rload16(rX, (unsigned)(rr * a.ldx + k0 + 4 * c4) * 4u);
         }
     };
     auto sstore = [&](auto S, int buf) {
@@ -300,17 +317,19 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
 #pragma unroll
         for (int u = 0; u < T::LA; ++u) {
             int q = tid + u * NTHR;
-            if (q < MT * KC / 4) {
+            if ((MT * KC / 4) % NTHR == 0 || q < MT * KC / 4) {
                 int rr = q / (KC / 4), c4 = q % (KC / 4);
-                *(v4f *)(As + (buf * MT + rr) * LD + 4 * c4) = R.a[u];
+                *(u32x4 *)(As + (buf * MT + rr) * LD + 4 * c4) = R.a[u];
             }
         }
 #pragma unroll
         for (int u = 0; u < T::LB; ++u) {
             int q = tid + u * NTHR;
-            if (q < NT * KC / 4) {
+            if ((NT * KC / 4) % NTHR == 0 || q < NT * KC / 4) {
                 int rr = q / (KC / 4), c4 = q % (KC / 4);
-                *(v4f *)(Bs + (buf * NT + rr) * LD + 4 * c4) = R.b[u];
+                v4f v = __builtin_bit_cast(v4f, R.b[u]);
+                if (sgn) v = (v4f){sgn1(v[0]), sgn1(v[1]), sgn1(v[2]), sgn1(v[3])};
+                *(v4f *)(Bs + (buf * NT + rr) * LD + 4 * c4) = v;
             }
         }
     };
@@ -400,11 +419,21 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     const int Kp = round_up_dev(a.N, KC);
     const unsigned short *Ph = (const unsigned short *)a.B;
     const unsigned short *Pl = Ph + a.bplane;
-    const u32x4 z4 = {0u, 0u, 0u, 0u};
+    // Rows row0.. of J and replicas col0.. of the B source; loads past N or R
+    // read zeros.
+    constexpr int JBYTES = FP8J ? 1 : 2;
+    const int rows = min(MT, a.N - row0), cols = min(NT, a.R - col0);
+    const __amdgpu_buffer_rsrc_t rJ = tile_rsrc((const unsigned char *)a.J + (size_t)row0 * a.ldj * JBYTES,
+                                                (long long)rows * a.ldj * JBYTES);
+    const __amdgpu_buffer_rsrc_t rBh =
+        PLANES ? tile_rsrc(Ph + (size_t)col0 * a.ldx, (long long)cols * a.ldx * 2)
+               : tile_rsrc(a.X + (size_t)col0 * a.ldx, (long long)cols * a.ldx * 4);
+    const __amdgpu_buffer_rsrc_t rBl =
+        tile_rsrc(PLANES && !SGN ? Pl + (size_t)col0 * a.ldx : Ph, PLANES && !SGN ? (long long)cols * a.ldx * 2 : 0);
 
     struct Stage {
         u32x4 ra[LA];
-        v4f rb[PLANES ? 1 : LB];
+        u32x4 rb[PLANES ? 1 : LB];
         u32x4 rbh[PLANES ? LB : 1], rbl[PLANES ? LB : 1];
     };
     Stage st0, st1;
@@ -413,35 +442,18 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
         const int k0 = ch * KC;
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
-            int q = tid + u * NTHR;
-            int rr = q / (KC / A_PER), c = q % (KC / A_PER);
-            int gi = row0 + rr;
-            u32x4 v = z4;
-            if (q < A_LOADS && gi < a.N) {
-                if (FP8J)
-                    v = *(const u32x4 *)((const unsigned char *)a.J + (size_t)gi * a.ldj + k0 + 16 * c);
-                else
-                    v = *(const u32x4 *)((const unsigned short *)a.J + (size_t)gi * a.ldj + k0 + 8 * c);
-            }
-            R.ra[u] = v;
+            const int q = tid + u * NTHR, rr = q / (KC / A_PER), c = q % (KC / A_PER);
+            R.ra[u] = rload16(rJ, (unsigned)(rr * a.ldj + k0 + A_PER * c) * JBYTES);
         }
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
-            int q = tid + u * NTHR;
-            int rr = q / (KC / B_PER), c = q % (KC / B_PER);
-            int gr = col0 + rr;
-            const bool ok = q < B_LOADS && gr < a.R;
+            const int q = tid + u * NTHR, rr = q / (KC / B_PER), c = q % (KC / B_PER);
             if (PLANES) {
-                const size_t o = (size_t)gr * a.ldx + k0 + 8 * c;
-                R.rbh[u] = z4;
-                R.rbl[u] = z4;
-                if (ok) {
-                    R.rbh[u] = *(const u32x4 *)(Ph + o);
-                    if (!SGN) R.rbl[u] = *(const u32x4 *)(Pl + o);
-                }
+                const unsigned o = (unsigned)(rr * a.ldx + k0 + 8 * c) * 2u;
+                R.rbh[u] = rload16(rBh, o);
+                if (!SGN) R.rbl[u] = rload16(rBl, o);
             } else {
-                R.rb[u] = (v4f){0.f, 0.f, 0.f, 0.f};
-                if (ok) R.rb[u] = *(const v4f *)(a.X + (size_t)gr * a.ldx + k0 + 4 * c);
+                R.rb[u] = rload16(rBh, (unsigned)(rr * a.ldx + k0 + 4 * c) * 4u);
             }
         }
     };
@@ -450,7 +462,7 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
             int q = tid + u * NTHR;
-            if (q < A_LOADS) {
+            if (A_LOADS % NTHR == 0 || q < A_LOADS) {
                 int rr = q / (KC / A_PER), c = q % (KC / A_PER);
                 unsigned short *dst = As + (buf * MT + rr) * LD + A_PER * c;
                 if (FP8J) {
@@ -467,16 +479,17 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
-            if (q < B_LOADS) {
+            if (B_LOADS % NTHR == 0 || q < B_LOADS) {
                 int rr = q / (KC / B_PER), c = q % (KC / B_PER);
                 if (PLANES) {
                     *(u32x4 *)(Bh + (buf * NT + rr) * LD + 8 * c) = R.rbh[u];
                     if (!SGN) *(u32x4 *)(Bl + (buf * NT + rr) * LD + 8 * c) = R.rbl[u];
                 } else {
                     u16x4 h, l;
+                    const v4f xv = __builtin_bit_cast(v4f, R.rb[u]);
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        float v = R.rb[u][e];
+                        float v = xv[e];
                         if (SGN) {
                             h[e] = v > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
                             l[e] = 0;
@@ -585,6 +598,14 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
     const int Kp = round_up_dev(a.N, 64);
     const unsigned char *P = (const unsigned char *)a.B;
 
+    // Rows row0.. of J and replicas col0.. of the B source; loads past N or R,
+    // and K columns at or past Kp (the last chunk can run past ld), read zeros.
+    const int rows = min(MT, a.N - row0), cols = min(NT, a.R - col0);
+    const __amdgpu_buffer_rsrc_t rJ =
+        tile_rsrc((const unsigned char *)a.J + (size_t)row0 * a.ldj, (long long)rows * a.ldj);
+    const __amdgpu_buffer_rsrc_t rB = PLANES ? tile_rsrc(P + (size_t)col0 * a.ldx, (long long)cols * a.ldx)
+                                             : tile_rsrc(a.X + (size_t)col0 * a.ldx, (long long)cols * a.ldx * 4);
+
     struct Stage {
         u32x4 ra[LA];
         unsigned int rb[PLANES ? 1 : LB];
@@ -596,31 +617,24 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
         const int k0 = ch * KC;
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
-            int q = tid + u * NTHR;
-            int rr = q / (KC / 16), c = q % (KC / 16);
-            int gi = row0 + rr, gk = k0 + 16 * c;
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (q < A_LOADS && gi < a.N && gk < Kp)
-                v = *(const u32x4 *)((const unsigned char *)a.J + (size_t)gi * a.ldj + gk);
-            R.ra[u] = v;
+            const int q = tid + u * NTHR, rr = q / (KC / 16), c = q % (KC / 16), gk = k0 + 16 * c;
+            R.ra[u] = rload16(rJ, gk < Kp ? (unsigned)(rr * a.ldj + gk) : kOutOfRange);
         }
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
-            int q = tid + u * NTHR;
-            int rr = q / (KC / B_PER), c = q % (KC / B_PER);
-            int gr = col0 + rr, gk = k0 + B_PER * c;
+            const int q = tid + u * NTHR, rr = q / (KC / B_PER), c = q % (KC / B_PER), gk = k0 + B_PER * c;
             if (PLANES) {
-                u32x4 v = {0u, 0u, 0u, 0u};
-                if (q < B_LOADS && gr < a.R && gk < Kp) v = *(const u32x4 *)(P + (size_t)gr * a.ldx + gk);
-                R.rbp[u] = v;
+                R.rbp[u] = rload16(rB, gk < Kp ? (unsigned)(rr * a.ldx + gk) : kOutOfRange);
             } else {
-                unsigned int s4 = 0u;  // FP8 zero bytes beyond N: J is zero there anyway
-                if (q < B_LOADS && gr < a.R && gk < Kp) {
-                    v4f v = *(const v4f *)(a.X + (size_t)gr * a.ldx + gk);
-                    // E4M3 +1.0 = 0x38, -1.0 = 0xB8
+                // Without planes (operator calls only) the sign bytes are packed
+                // at once, which keeps the stage small.  E4M3 +1.0 = 0x38,
+                // -1.0 = 0xB8; zeros read past the range become -1 (J is zero
+                // there).
+                const v4f v = __builtin_bit_cast(v4f, rload16(rB, gk < Kp ? (unsigned)(rr * a.ldx + gk) * 4u
+                                                                         : kOutOfRange));
+                unsigned int s4 = 0u;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) s4 |= (v[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
-                }
+                for (int e = 0; e < 4; ++e) s4 |= (v[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
                 R.rb[u] = s4;
             }
         }
@@ -630,7 +644,7 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 #pragma unroll
         for (int u = 0; u < LA; ++u) {
             int q = tid + u * NTHR;
-            if (q < A_LOADS) {
+            if (A_LOADS % NTHR == 0 || q < A_LOADS) {
                 int rr = q / (KC / 16), c = q % (KC / 16);
                 *(u32x4 *)(As + (buf * MT + rr) * LD + 16 * c) = R.ra[u];
             }
@@ -638,7 +652,7 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             int q = tid + u * NTHR;
-            if (q < B_LOADS) {
+            if (B_LOADS % NTHR == 0 || q < B_LOADS) {
                 int rr = q / (KC / B_PER), c = q % (KC / B_PER);
                 if (PLANES)
                     *(u32x4 *)(Bs + (buf * NT + rr) * LD + 16 * c) = R.rbp[u];

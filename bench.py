@@ -63,6 +63,19 @@ def solve_timed(ctx, prob, K, W, **kw):
     return wall, last
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the kernel this workload runs, from the committed
+    rocprofv3 counter summary (profiles/pmc_traffic.json), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        w, cal = t["workloads"][workload], t["calibration"]
+    except (OSError, KeyError, ValueError):
+        return None
+    return (w["fetch_kib"] * cal["fetch_scale"] + w["write_kib"] * cal["write_scale"]) * 1024.0
+
+
 def leg_precision(sb, ctx, S):
     N3, R3 = 2000, 4096
     p3 = ctx.problem(sb.generate(sb.GEN_KN, N3, 1.0, seed=3))
@@ -85,12 +98,15 @@ def leg_precision(sb, ctx, S):
     leg["mean_energy_gap_split_vs_fp32_bsb"] = (s["mean_energy"] - f["mean_energy"]) / abs(f["mean_energy"])
     # split bSB issues two BF16 MFMA passes: 2 x the algorithmic flops
     leg["split_bsb_roofline"] = {"bound": "mfma", "achieved_issued": 2 * s["alg_tflops"], "peak": PEAK_BF16_TFLOPS,
-                                 "unit": "TFLOP/s", "frac": 2 * s["alg_tflops"] / PEAK_BF16_TFLOPS}
-    # dSB FP8: state traffic per step, 16 B/spin-replica (x, y read+write) + 1 B sign plane written
-    st_bytes = (16 + 1) * N3 * R3 + N3 * N3
+                                 "unit": "TFLOP/s", "frac": 2 * s["alg_tflops"] / PEAK_BF16_TFLOPS,
+                                 "traffic": measured_traffic("bSB K2000 R=4096 split")}
+    # dSB FP8: state traffic per step, 16 B/spin-replica (x, y read+write) + the 1 B sign plane read
+    # and the next one written, + J (FP8) once
+    st_bytes = (16 + 2) * N3 * R3 + N3 * N3
     gbs = st_bytes / (leg["split_dsb"]["avg_launch_us"] * 1e-6) / 1e9
     leg["split_dsb_roofline"] = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                 "frac": gbs / PEAK_HBM_GBS, "bytes_per_launch": st_bytes}
+                                 "frac": gbs / PEAK_HBM_GBS, "bytes_per_launch": st_bytes,
+                                 "traffic": measured_traffic("dSB K2000 R=4096 split")}
     leg["workload"] = "K2000 +-1 (FP8-exact J), R=4096, 2000 steps, dt=1.0, seed 3"
     p3.close()
     return leg
@@ -343,7 +359,8 @@ def main() -> int:
                                "FP32 plain path, one solve per step",
                    "N": N, "replicas": R, "sb_steps": S, "variant": "bsb", "precision": "fp32"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic("bSB SK N=2000 R=1024 fp32"),
+                     "traffic_algorithmic": 4.0 * N * N + 16.0 * N * R,
                      "kernel": "k_step_fp32 (FP32 MFMA 32x32x2, fused bSB update)",
                      "avg_launch_us": avg_launch_s * 1e6, "flops_per_launch": flops},
         "best_energy": float(last.energy.min()),

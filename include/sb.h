@@ -158,8 +158,8 @@ SB_API int sb_generate(int32_t kind, int32_t N, double p, uint64_t seed, float *
 SB_API int sb_ctx_create(int32_t device, sb_ctx **out);
 /* Fails with SB_E_INVALID while problems created on ctx are still alive. */
 SB_API int sb_ctx_destroy(sb_ctx *ctx);
-/* Upload a dense host J (FP32, [N][ldj]); stores FP32 and, when exact, FP8
- * (integer |J| <= 16) else BF16 forms on the device. */
+/* Upload a dense host J (FP32, [N][ldj]); stores FP32 and BF16 forms on the
+ * device, and an FP8 E4M3 form when exact (integer |J| <= 16). */
 SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, int32_t ldj,
                              sb_problem **out);
 SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *path, int one_based,

@@ -1022,7 +1022,7 @@ hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
 // single-instance kernel of the same family and shape runs, so an instance's
 // result does not depend on what it is batched with.
 
-#define SB_BATCH_FP32(X) X(2, 2, 1, 1) X(2, 2, 2, 1) X(4, 1, 1, 1) X(1, 4, 1, 1)
+#define SB_BATCH_FP32(X) X(2, 2, 1, 1) X(2, 2, 2, 1) X(4, 1, 1, 1) X(1, 4, 1, 1) X(2, 2, 1, 2)
 #define SB_BATCH_SPLIT(X) X(2, 2, 1, 1) X(2, 2, 2, 1) X(4, 1, 1, 1)
 #define SB_BATCH_FP8(X) X(2, 2, 1, 1) X(2, 2, 2, 1) X(4, 1, 1, 1)
 
@@ -1033,7 +1033,37 @@ static const TileShape kBatchFp8[] = {SB_BATCH_FP8(SB_SHAPE_ENTRY)};
 static const TileShape kBatchThin[] = {SB_THIN_ENTRY(1) SB_THIN_ENTRY(2) SB_THIN_ENTRY(4)};
 #undef SB_SHAPE_ENTRY
 
-constexpr size_t kBatchLds = 73728;  // max over the shapes above (split 128x64)
+constexpr size_t kBatchLds = 73728;  // max over the shapes above (checked below)
+
+// LDS bytes of one tile of each family (the layouts of fp32_tile, split_tile
+// and dsb_fp8_tile): every batched shape must fit the dynamic allocation.
+template <int WM, int WN, int BM, int BN>
+constexpr size_t fp32_lds()
+{
+    using T = Fp32Tile<WM, WN, BM, BN>;
+    return 2 * (size_t)(T::MT + T::NT) * T::LDS_LD * sizeof(float);
+}
+template <int WM, int WN, int BM, int BN>
+constexpr size_t split_lds()
+{
+    using T = SplitTile<WM, WN, BM, BN>;
+    return 2 * (size_t)(T::MT + 2 * T::NT) * T::LDS_LD * sizeof(unsigned short);
+}
+template <int WM, int WN, int BM, int BN>
+constexpr size_t fp8_lds()
+{
+    using T = Fp8Tile<WM, WN, BM, BN>;
+    return 2 * (size_t)(T::MT + T::NT) * T::LDS_LD;
+}
+#define SB_CHECK_FP32(wm, wn, bm, bn) static_assert(fp32_lds<wm, wn, bm, bn>() <= kBatchLds, "batched FP32 LDS");
+#define SB_CHECK_SPLIT(wm, wn, bm, bn) static_assert(split_lds<wm, wn, bm, bn>() <= kBatchLds, "batched split LDS");
+#define SB_CHECK_FP8(wm, wn, bm, bn) static_assert(fp8_lds<wm, wn, bm, bn>() <= kBatchLds, "batched FP8 LDS");
+SB_BATCH_FP32(SB_CHECK_FP32)
+SB_BATCH_SPLIT(SB_CHECK_SPLIT)
+SB_BATCH_FP8(SB_CHECK_FP8)
+#undef SB_CHECK_FP32
+#undef SB_CHECK_SPLIT
+#undef SB_CHECK_FP8
 
 int batch_shape_count(int family)
 {
@@ -1115,6 +1145,8 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         case (FAM_FP32 * 2 + 1) * 8 + 2: fp32_tile<4, 1, 1, 1>(a, item.tile, (float *)smem_dyn); break;
         case (FAM_FP32 * 2 + 0) * 8 + 3:
         case (FAM_FP32 * 2 + 1) * 8 + 3: fp32_tile<1, 4, 1, 1>(a, item.tile, (float *)smem_dyn); break;
+        case (FAM_FP32 * 2 + 0) * 8 + 4:
+        case (FAM_FP32 * 2 + 1) * 8 + 4: fp32_tile<2, 2, 1, 2>(a, item.tile, (float *)smem_dyn); break;
         case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, false, true, false>(a, item.tile, sm16); break;
         case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, false, true, false>(a, item.tile, sm16); break;
         case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, false, true, false>(a, item.tile, sm16); break;

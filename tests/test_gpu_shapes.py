@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 # Compiled shapes per family (csrc/sb_kernels.hip: SB_*_SHAPES, SB_BATCH_*).
 SHAPES = {0: 13, 1: 10, 2: 10, 3: 11}
-BATCH_SHAPES = {0: 4, 1: 3, 2: 3, 3: 3}
+BATCH_SHAPES = {0: 5, 1: 3, 2: 3, 3: 3}
 N, R, STEPS = 300, 40, 30
 
 

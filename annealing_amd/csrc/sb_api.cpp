@@ -215,7 +215,7 @@ extern "C" SB_API int sb_instance_stats(int32_t N, const float *J, int32_t ldj, 
 extern "C" SB_API double sb_c0_auto(int32_t variant, int32_t N, double a0, double sigma)
 {
     if (N < 1 || !(sigma > 0.0)) return 0.0;
-    if (variant == SB_ASB) return 0.7 * a0 / (sigma * std::sqrt((double)N));
+    if (variant == SB_ASB || variant == SB_ASB_M1) return 0.7 * a0 / (sigma * std::sqrt((double)N));
     return 0.5 / (std::sqrt((double)N) * sigma);
 }
 
@@ -546,10 +546,24 @@ extern "C" SB_API int sb_problem_read_j(const sb_problem *p, float *J, int32_t l
 
 namespace sbh {
 
+bool has_prologue(int variant) { return variant == SB_ASB || variant == SB_ASB_M1; }
+
+sb_step_coeffs prologue_coeffs(const sb_params &p)
+{
+    sb_step_coeffs c{};
+    c.variant = p.variant;
+    c.a0 = p.a0;
+    c.K = p.K;
+    c.M = p.M;
+    c.dts = p.variant == SB_ASB_M1 ? p.dt : p.dt / (double)p.M;
+    c.p_next = 0.0f;
+    return c;
+}
+
 int check_params(const sb_params *p, int N, double sigma, double *c0_out)
 {
     if (!p) return SB_E_INVALID;
-    if (p->variant < SB_ASB || p->variant > SB_DSB) return SB_E_INVALID;
+    if (p->variant < SB_ASB || p->variant > SB_ASB_M1) return SB_E_INVALID;
     if (p->precision < SB_PREC_FP32 || p->precision > SB_PREC_ORDERED) return SB_E_INVALID;
     if (p->init != SB_INIT_UNIFORM && p->init != SB_INIT_REFERENCE) return SB_E_INVALID;
     if (p->schedule != SB_SCHED_LINEAR) return SB_E_INVALID;
@@ -825,16 +839,7 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     if (rc) return rc;
     float *X = ctx->X0, *Xn = ctx->X1, *Y = ctx->Y;
     HIP_TRY(launch_init(p->init, p->seed, N, R, ld, X, Y, Xn, s));
-    sb_step_coeffs c{};
-    if (p->variant == SB_ASB) {
-        // Local sub-steps of step 0 (m2_opt.cu:168, s = 0).
-        c.a0 = p->a0;
-        c.K = p->K;
-        c.dts = p->dt / (double)p->M;
-        c.M = p->M;
-        c.p_next = 0.0f;
-        HIP_TRY(launch_asb_local(c, N, R, ld, X, Y, s));
-    }
+    if (has_prologue(p->variant)) HIP_TRY(launch_asb_local(prologue_coeffs(*p), N, R, ld, X, Y, s));
     StepArgs a{};
     a.J = p->precision == SB_PREC_ORDERED ? (const void *)prob->J32 : family_J(family, prob);
     a.ldj = ld;
@@ -903,7 +908,7 @@ extern "C" SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, in
 {
     if (!ctx || !dJ || !dX || !dXn || !dY || !c || N < 1 || R < 1) return SB_E_INVALID;
     if (ldj < N || ldx < N || ldj % kSpinPad || ldx % kSpinPad) return SB_E_INVALID;
-    if (c->variant < SB_ASB || c->variant > SB_DSB) return SB_E_INVALID;
+    if (c->variant < SB_ASB || c->variant > SB_ASB_M1) return SB_E_INVALID;
     int family;
     const bool thin = R <= kThinMaxR;
     if (precision == SB_PREC_FP32 || precision == SB_PREC_ORDERED) {
@@ -944,7 +949,7 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
                                int32_t R, float *F)
 {
     if (!ctx || !prob || !X || !F || R < 1 || prob->ctx != ctx) return SB_E_INVALID;
-    if (variant < SB_ASB || variant > SB_DSB) return SB_E_INVALID;
+    if (variant < SB_ASB || variant > SB_ASB_M1) return SB_E_INVALID;
     if (precision < SB_PREC_FP32 || precision > SB_PREC_ORDERED) return SB_E_INVALID;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
@@ -968,7 +973,8 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
     a.ldx = ld;
     a.N = N;
     a.R = R;
-    a.c.variant = variant;
+    // aSB-m1 has the same field J.x; aSB's kick with beta = 1 returns it.
+    a.c.variant = variant == SB_ASB_M1 ? SB_ASB : variant;
     a.c.beta = 1.0f;
     a.c.do_local = 0;
     hipError_t e;
@@ -1010,16 +1016,11 @@ extern "C" SB_API int sb_asb_prologue(sb_ctx *ctx, const sb_params *p, int32_t N
                                       float *dX, float *dY, void *stream)
 {
     if (!ctx || !p || !dX || !dY || N < 1 || R < 1 || ldx < N || p->M < 1) return SB_E_INVALID;
+    if (!sbh::has_prologue(p->variant)) return SB_E_INVALID;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    sb_step_coeffs c{};
-    c.a0 = p->a0;
-    c.K = p->K;
-    c.dts = p->dt / (double)p->M;
-    c.M = p->M;
-    c.p_next = 0.0f;
-    HIP_TRY(launch_asb_local(c, N, R, ldx, dX, dY, s));
+    HIP_TRY(launch_asb_local(sbh::prologue_coeffs(*p), N, R, ldx, dX, dY, s));
     return SB_OK;
 }
 

@@ -18,7 +18,8 @@ static void usage()
     std::printf(
         "Usage:\n"
         "       sb_solve <instance> [options]\n"
-        "  --variant asb|bsb|dsb     update rule (default asb, the reference's)\n"
+        "  --variant asb|asb-m1|bsb|dsb  update rule (default asb, m2_opt.cu; asb-m1 is\n"
+        "                            Bifurcation_m1.cu.c)\n"
         "  --precision fp32|split|ordered\n"
         "  --steps N                 (default 100)\n"
         "  --dt X                    coupling step DeltaT (default 1.0)\n"
@@ -82,6 +83,7 @@ int main(int argc, char **argv)
             if (v == "asb") p.variant = SB_ASB;
             else if (v == "bsb") p.variant = SB_BSB;
             else if (v == "dsb") p.variant = SB_DSB;
+            else if (v == "asb-m1") p.variant = SB_ASB_M1;
             else { std::fprintf(stderr, "bad --variant %s\n", v.c_str()); return 2; }
         } else if (a == "--precision") {
             std::string v = need("--precision");

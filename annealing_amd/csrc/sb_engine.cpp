@@ -103,15 +103,7 @@ int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
     s.used = true;
     hipStream_t st = ctx_->stream;
     HIP_TRY(launch_init(p.init, p.seed, N, R, ld, s.X[0], s.Y, s.X[1], st));
-    if (p.variant == SB_ASB) {
-        sb_step_coeffs c{};
-        c.a0 = p.a0;
-        c.K = p.K;
-        c.dts = p.dt / (double)p.M;
-        c.M = p.M;
-        c.p_next = 0.0f;
-        HIP_TRY(launch_asb_local(c, N, R, ld, s.X[0], s.Y, st));
-    }
+    if (has_prologue(p.variant)) HIP_TRY(launch_asb_local(prologue_coeffs(p), N, R, ld, s.X[0], s.Y, st));
     // Both plane buffers start complete (the padding columns are read by the
     // K loop; the epilogue rewrites only groups holding real spins).
     if (s.bkind != BK_NONE) {

@@ -52,6 +52,11 @@ int hip_fail(hipError_t e, const char *what);
 
 // Validate params against a problem; fills the derived c0.
 int check_params(const sb_params *p, int N, double sigma, double *c0_out);
+// aSB forms: the coefficients of step 0's local update, applied before the
+// first fused step (aSB: M sub-steps, m2_opt.cu:168 with s = 0; aSB-m1: the
+// x-step, Bifurcation_m1.cu.c:166).  has_prologue is false for bSB/dSB.
+bool has_prologue(int variant);
+sb_step_coeffs prologue_coeffs(const sb_params &p);
 
 // Pick the kernel family for a precision/variant/problem (before tuning).
 // Returns the list of families whose results are identical for this case;

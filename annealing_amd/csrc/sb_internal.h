@@ -220,7 +220,15 @@ inline sb_step_coeffs step_coeffs(int variant, int k, int steps, double dt, doub
     c.a0 = a0;
     c.K = K;
     c.dts = M > 0 ? dt / (double)M : 0.0;
-    if (variant == SB_ASB) {
+    if (variant == SB_ASB_M1) {
+        // Bifurcation_m1.cu.c:166-169: step k is UpdateX, then UpdateY with
+        // amplitude k/(float)STEP; the fused kernel applies the y-step of step
+        // k and then the x-step of step k+1 (a prologue does step 0's).
+        c.dts = dt;
+        c.c0 = c0;
+        c.p_next = (float)k / (float)steps;
+        c.do_local = (k + 1 < steps) ? 1 : 0;
+    } else if (variant == SB_ASB) {
         c.beta = (float)(dt * c0);  // m2_opt.cu:148 coeff1 = DeltaT * xi
         c.do_local = (k + 1 < steps) ? 1 : 0;
         c.p_next = (k + 1 < steps) ? (float)(k + 1) / (float)steps : 0.0f;  // m2_opt.cu:158

@@ -77,9 +77,37 @@ __device__ __forceinline__ void asb_local(float &x, float &y, float p, double a0
     }
 }
 
+// Bifurcation_m1.cu.c:55-69 (UpdateX): detune * y in FP32, then the product
+// with DeltaT and the sum in FP64, stored as FP32.
+__device__ __forceinline__ float m1_x(float x, float y, double a0, double dt)
+{
+    const float t = (float)a0 * y;
+    return (float)((double)t * dt + (double)x);
+}
+
+// Bifurcation_m1.cu.c:71-92 (UpdateY): x^3, p x and detune x in FP32, each
+// times DeltaT and summed in FP64 ((A + B) - C), stored as FP32; the coupling
+// sum times xi and DeltaT in FP64, stored as FP32; then two FP32 additions.
+__device__ __forceinline__ float m1_y(float F, float x, float y, float p, double a0, double K, double dt,
+                                      double xi)
+{
+    const float x3 = x * x * x;
+    const double A = (double)(-(float)K * x3) * dt;
+    const double B = (double)(p * x) * dt;
+    const double C = (double)((float)a0 * x) * dt;
+    float ny = (float)((A + B) - C);
+    const float s = (float)(((double)F * xi) * dt);
+    ny = ny + s;
+    ny = ny + y;
+    return ny;
+}
+
 __device__ __forceinline__ void sb_update(const sb_step_coeffs &c, float F, float &x, float &y)
 {
-    if (c.variant == SB_ASB) {
+    if (c.variant == SB_ASB_M1) {
+        y = m1_y(F, x, y, c.p_next, c.a0, c.K, c.dts, c.c0);
+        if (c.do_local) x = m1_x(x, y, c.a0, c.dts);
+    } else if (c.variant == SB_ASB) {
         y = __builtin_fmaf(c.beta, F, y);  // cublasSgemv(alpha = dt xi, beta = 1)
         if (c.do_local) asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
     } else {
@@ -1384,7 +1412,10 @@ __global__ __launch_bounds__(256) void k_asb_local(sb_step_coeffs c, int N, int 
     if (i >= N || r >= R) return;
     size_t o = (size_t)r * ldx + i;
     float x = X[o], y = Y[o];
-    asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
+    if (c.variant == SB_ASB_M1)
+        x = m1_x(x, y, c.a0, c.dts);  // step 0's x-step
+    else
+        asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
     X[o] = x;
     Y[o] = y;
 }

@@ -19,13 +19,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsbamd.so")
 
-ASB, BSB, DSB = 0, 1, 2
+ASB, BSB, DSB, ASB_M1 = 0, 1, 2, 3
 PREC_FP32, PREC_SPLIT, PREC_ORDERED = 0, 1, 2
 INIT_UNIFORM, INIT_REFERENCE = 0, 1
 J_FP32, J_BF16, J_FP8 = 0, 1, 2
 GEN_ER, GEN_SK, GEN_KN, GEN_SK_DILUTED = 0, 1, 2, 3
 
-VARIANTS = {"asb": ASB, "bsb": BSB, "dsb": DSB}
+VARIANTS = {"asb": ASB, "bsb": BSB, "dsb": DSB, "asb_m1": ASB_M1}
 PRECISIONS = {"fp32": PREC_FP32, "split": PREC_SPLIT, "ordered": PREC_ORDERED}
 INITS = {"uniform": INIT_UNIFORM, "reference": INIT_REFERENCE}
 
@@ -78,7 +78,7 @@ class StepCoeffs(C.Structure):
         ("variant", C.c_int32), ("do_local", C.c_int32), ("M", C.c_int32),
         ("reserved", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float),
         ("gamma", C.c_float), ("p_next", C.c_float), ("a0", C.c_double),
-        ("K", C.c_double), ("dts", C.c_double),
+        ("K", C.c_double), ("dts", C.c_double), ("c0", C.c_double),
     ]
 
 

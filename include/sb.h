@@ -55,10 +55,14 @@ typedef enum sb_status {
 } sb_status;
 
 /* Update rule.  SB_ASB is the reference's adiabatic SB with Kerr term and M
- * local sub-steps per coupling kick (m2_opt.cu:58-75,168-169).  SB_BSB and
- * SB_DSB are ballistic and discrete SB (Goto et al., Sci. Adv. 7:eabe7953,
- * 2021), which the reference does not implement. */
-typedef enum sb_variant { SB_ASB = 0, SB_BSB = 1, SB_DSB = 2 } sb_variant;
+ * local sub-steps per coupling kick (m2_opt.cu:58-75,168-169).  SB_ASB_M1 is
+ * the reference's one-x-step, one-y-step variant with the coupling inside the
+ * y-step (Bifurcation_m1.cu.c:55-92,166-169): per step x += a0 y dt, then
+ * y += dt (-K x^3 + p x - a0 x) + xi dt (J x) with the new x, in the
+ * reference's mixed FP32/FP64 arithmetic.  SB_BSB and SB_DSB are ballistic and
+ * discrete SB (Goto et al., Sci. Adv. 7:eabe7953, 2021), which the reference
+ * does not implement. */
+typedef enum sb_variant { SB_ASB = 0, SB_BSB = 1, SB_DSB = 2, SB_ASB_M1 = 3 } sb_variant;
 
 /* Arithmetic of the field F = J.X.
  *   SB_PREC_FP32    FP32 J and X on FP32 MFMA (the plain path).
@@ -183,11 +187,13 @@ SB_API int sb_solve(sb_ctx *ctx, sb_problem *prob, const sb_params *params,
 /* Coefficients of one fused step (see sb_step_coeffs_for). */
 typedef struct sb_step_coeffs {
     int32_t variant;
-    int32_t do_local; /* aSB: apply the next step's local sub-steps after the kick */
+    int32_t do_local; /* aSB, aSB-m1: apply the next step's local (x) update after the kick */
     int32_t M;
     int32_t reserved;
-    float alpha, beta, gamma, p_next;
+    float alpha, beta, gamma;
+    float p_next;     /* aSB: pump of the next step's sub-steps; aSB-m1: pump of this y-step */
     double a0, K, dts;
+    double c0;        /* aSB-m1: the coupling xi, applied in FP64 */
 } sb_step_coeffs;
 
 /* Fill the coefficients of fused step k of an S-step schedule. */

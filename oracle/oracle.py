@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-ASB, BSB, DSB = 0, 1, 2
+ASB, BSB, DSB, ASB_M1 = 0, 1, 2, 3
 ORDER_SEQ, ORDER_LANES = 0, 1  # field summation orders (sb_oracle.h)
 
 _lib = None
@@ -39,6 +39,8 @@ def load():
             "orc_field_f32_lanes": (None, [I, I, P, I, P, I, I, P]),
             "orc_bsb_update_f32": (None, [F, F, F, F, C.POINTER(F), C.POINTER(F)]),
             "orc_asb_local_f32": (None, [C.POINTER(F), C.POINTER(F), F, D, D, D, I]),
+            "orc_m1_x_f32": (F, [F, F, D, D]),
+            "orc_m1_y_f32": (F, [F, F, F, F, D, D, D, D]),
             "orc_run_f32": (I, [I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
             "orc_run_f64": (I, [I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
             "orc_run_f32_order": (I, [I, I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
@@ -139,6 +141,14 @@ def asb_local_f32(x, y, p, a0, K, dts, M):
     xx, yy = C.c_float(x), C.c_float(y)
     load().orc_asb_local_f32(C.byref(xx), C.byref(yy), p, a0, K, dts, M)
     return xx.value, yy.value
+
+
+def m1_x_f32(x, y, a0, dt):
+    return load().orc_m1_x_f32(x, y, a0, dt)
+
+
+def m1_y_f32(F, x, y, p, a0, K, dt, xi):
+    return load().orc_m1_y_f32(F, x, y, p, a0, K, dt, xi)
 
 
 def run_f32(variant, J, x, y, k0, k1, steps, dt=1.0, a0=1.0, c0=0.0, K=1.0, M=2, nthreads=1,

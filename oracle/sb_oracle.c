@@ -110,13 +110,18 @@ double orc_c0_auto(int variant, int N, double a0, const float *J, int ldj)
 {
     double sg = orc_sigma(N, J, ldj);
     if (sg <= 0.0) return 0.0;
-    if (variant == ORC_ASB) return 0.7 * a0 / (sg * sqrt((double)N));
+    if (variant == ORC_ASB || variant == ORC_ASB_M1) return 0.7 * a0 / (sg * sqrt((double)N));
     return 0.5 / (sqrt((double)N) * sg);
 }
 
 void orc_coeffs(int variant, int k, int steps, double dt, double a0, double c0,
                 float out[4])
 {
+    if (variant == ORC_ASB_M1) {
+        out[0] = out[1] = out[2] = 0.0f;
+        out[3] = (float)k / (float)steps; /* Bifurcation_m1.cu.c:165 amplitude */
+        return;
+    }
     if (variant == ORC_ASB) {
         float p = (float)k / (float)steps; /* m2_opt.cu:158 */
         out[0] = 0.0f;
@@ -314,6 +319,44 @@ static void asb_local_f64(double *x, double *y, double p, double a0, double K,
     *y = ly;
 }
 
+/* SB/CUDA_GPU/Bifurcation_m1.cu.c:55-69 (UpdateX).  detune is an int and
+ * DeltaT a double literal: detune * y is an FP32 product, the rest is
+ * evaluated in FP64 and rounded to FP32 on assignment. */
+float orc_m1_x_f32(float x, float y, double a0, double dt)
+{
+    float t = (float)a0 * y;
+    return (float)((double)t * dt + (double)x);
+}
+
+/* SB/CUDA_GPU/Bifurcation_m1.cu.c:71-92 (UpdateY), with the coupling sum F
+ * given:
+ *   newY = -K * (x * x * x) * DeltaT + amplitude * x * DeltaT - detune * x * DeltaT;
+ *   s = s * xi * DeltaT;  newY += s;  newY += oldY;
+ * x * x * x, (-K) * that, amplitude * x and detune * x are FP32 products; each
+ * is then multiplied by DeltaT in FP64 and the three are summed in FP64 as
+ * (A + B) - C and rounded to FP32; s * xi * DeltaT is FP64, rounded to FP32;
+ * the two += are FP32 additions. */
+float orc_m1_y_f32(float F, float x, float y, float p, double a0, double K, double dt, double xi)
+{
+    float x3 = x * x;
+    x3 = x3 * x;
+    float kx3 = -(float)K * x3;
+    float px = p * x;
+    float dx = (float)a0 * x;
+    double A = (double)kx3 * dt;
+    double B = (double)px * dt;
+    double C = (double)dx * dt;
+    double u = A + B;
+    u = u - C;
+    float ny = (float)u;
+    double sd = (double)F * xi;
+    sd = sd * dt;
+    float s = (float)sd;
+    ny = ny + s;
+    ny = ny + y;
+    return ny;
+}
+
 /* ---- solver loops ------------------------------------------------------ */
 
 /* Field of one replica in the chosen order; s has round_up(N, 16) entries. */
@@ -346,7 +389,14 @@ static int run_f32_replica(int order, int variant, int N, const float *J, int ld
     for (int k = k0; k < k1; ++k) {
         float cf[4];
         orc_coeffs(variant, k, steps, dt, a0, c0, cf);
-        if (variant == ORC_ASB) {
+        if (variant == ORC_ASB_M1) {
+            /* Bifurcation_m1.cu.c:166-169: UpdateX over all spins, then
+             * UpdateY with the coupling of the new x. */
+            const float p = (float)k / (float)steps;
+            for (int i = 0; i < N; ++i) x[i] = orc_m1_x_f32(x[i], y[i], a0, dt);
+            field_order(order, ORC_BSB, N, J, ldj, x, s, F);
+            for (int i = 0; i < N; ++i) y[i] = orc_m1_y_f32(F[i], x[i], y[i], p, a0, K, dt, c0);
+        } else if (variant == ORC_ASB) {
             /* m2_opt.cu:168-169: local sub-steps, then the kick with new x */
             for (int i = 0; i < N; ++i) orc_asb_local_f32(&x[i], &y[i], cf[3], a0, K, dts, M);
             field_order(order, variant, N, J, ldj, x, s, F);
@@ -362,7 +412,7 @@ static int run_f32_replica(int order, int variant, int N, const float *J, int ld
 static int check_run_args(int variant, int N, int R, int ldj, int ld, int k0, int k1,
                           int steps, int M)
 {
-    if (variant < ORC_ASB || variant > ORC_DSB) return ORC_E_INVALID;
+    if (variant < ORC_ASB || variant > ORC_ASB_M1) return ORC_E_INVALID;
     if (N < 1 || R < 0 || ldj < N || ld < N || steps < 1) return ORC_E_INVALID;
     if (k0 < 0 || k1 < k0 || k1 > steps) return ORC_E_INVALID;
     if (variant == ORC_ASB && M < 1) return ORC_E_INVALID;
@@ -424,7 +474,15 @@ int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
             double *xr = x + (size_t)r * ld, *yr = y + (size_t)r * ld;
             for (int k = k0; k < k1; ++k) {
                 double p = (double)k / (double)steps;
-                if (variant == ORC_ASB) {
+                if (variant == ORC_ASB_M1) {
+                    /* the same step in FP64 throughout */
+                    for (int i = 0; i < N; ++i) xr[i] = xr[i] + a0 * yr[i] * dt;
+                    orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
+                    for (int i = 0; i < N; ++i) {
+                        const double xi_ = xr[i];
+                        yr[i] = (-K * xi_ * xi_ * xi_ * dt + p * xi_ * dt - a0 * xi_ * dt) + F[i] * c0 * dt + yr[i];
+                    }
+                } else if (variant == ORC_ASB) {
                     for (int i = 0; i < N; ++i) asb_local_f64(&xr[i], &yr[i], p, a0, K, dts, M);
                     orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
                     for (int i = 0; i < N; ++i) yr[i] = fma(dt * c0, F[i], yr[i]);

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-enum { ORC_ASB = 0, ORC_BSB = 1, ORC_DSB = 2 };
+enum { ORC_ASB = 0, ORC_BSB = 1, ORC_DSB = 2, ORC_ASB_M1 = 3 };
 /* Field summation orders: SEQ is the plain j = 0..N-1 fmaf chain per row;
  * LANES regroups it into 64 strided lane sums combined pairwise (the order of
  * the few-replica matrix-vector kernels; see orc_field_f32_lanes). */
@@ -95,6 +95,11 @@ void orc_field_f64(int N, int R, const float *J, int ldj, const double *x, int l
  * combined as p[l] = p[l] + p[l ^ m] for m = 32, 16, ..., 1 and p[0] taken. */
 void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, int ld,
                          int use_sign, float *F);
+
+/* aSB-m1 (Bifurcation_m1.cu.c:55-92): the x-step and the y-step with the
+ * coupling sum F of the new x, in the reference's FP32/FP64 mix. */
+float orc_m1_x_f32(float x, float y, double a0, double dt);
+float orc_m1_y_f32(float F, float x, float y, float p, double a0, double K, double dt, double xi);
 
 /* FP32 epilogues, exactly as the product defines them. */
 void orc_bsb_update_f32(float F, float alpha, float beta, float gamma,

@@ -27,7 +27,7 @@ MIX = [("er", 37), ("sk", 64), ("kn", 130), ("er", 257), ("sk", 600), ("kn", 96)
 @pytest.mark.parametrize("precision", ["fp32", "split"])
 def test_batch_equals_solo(precision):
     Js = make_mix(MIX)
-    variants = ["bsb", "dsb", "asb", "bsb", "bsb", "dsb", "bsb"]
+    variants = ["bsb", "dsb", "asb", "bsb", "asb_m1", "dsb", "bsb"]
     Rs = [5, 64, 33, 100, 16, 1, 64]
     steps = [40, 100, 73, 60, 50, 90, 30]
     with sb.Context(0) as ctx:

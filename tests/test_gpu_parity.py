@@ -60,7 +60,7 @@ def test_init_bit_exact(orc, init):
 
 # ---- ordered-sum mode: bit-exact trajectories ---------------------------------
 
-@pytest.mark.parametrize("variant", ["asb", "bsb", "dsb"])
+@pytest.mark.parametrize("variant", ["asb", "bsb", "dsb", "asb_m1"])
 @pytest.mark.parametrize("N,R,steps,init", [(67, 5, 120, "uniform"), (200, 12, 150, "reference")])
 def test_ordered_matches_oracle_bit_exact(ctx, orc, variant, N, R, steps, init):
     J = sk(N, 11) if variant == "bsb" else kn_pm1(N, 12)

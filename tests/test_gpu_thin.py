@@ -32,7 +32,7 @@ def solve_forced(ctx, prob, family, shape, **kw):
         del os.environ["SB_SHAPE"]
 
 
-@pytest.mark.parametrize("variant", ["bsb", "asb", "dsb"])
+@pytest.mark.parametrize("variant", ["bsb", "asb", "dsb", "asb_m1"])
 @pytest.mark.parametrize("N,R", [(300, 1), (1100, 3), (2100, 4)])
 def test_thin_fp32_bit_exact_vs_lane_order_oracle(ctx, orc, variant, N, R):
     J = sb.generate(sb.GEN_SK if variant == "bsb" else sb.GEN_KN, N, 1.0, 21)

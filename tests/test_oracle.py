@@ -218,6 +218,61 @@ def test_lane_order_run_equals_sequential_for_exact_fields(orc):
     assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
 
 
+# ---- R12 aSB-m1 (Bifurcation_m1.cu.c:55-92) ------------------------------------
+
+def _sk(N, seed):
+    rng = np.random.default_rng(seed)
+    J = (rng.standard_normal((N, N)) / np.sqrt(N)).astype(np.float32)
+    J = np.triu(J, 1)
+    return (J + J.T).astype(np.float32)
+
+
+def test_m1_fp64_equals_asb_one_substep(orc):
+    # x += a0 y dt, then y += dt (-K x^3 + p x - a0 x) + xi dt F(x) is aSB with
+    # one local sub-step and the kick on the same x: in FP64 the two code paths
+    # (aSB's is pinned by the recorded trajectory) must agree to rounding.
+    N, R, steps = 64, 3, 120
+    J = _sk(N, 5)
+    x0, y0 = orc.init_state("uniform", N, R, seed=3)
+    xa, ya = x0.astype(np.float64), y0.astype(np.float64)
+    xb, yb = xa.copy(), ya.copy()
+    c0 = orc.c0_auto(orc.ASB, J)
+    orc.run_f64(orc.ASB_M1, J, xa, ya, 0, steps, steps, 0.5, 1.0, c0, 1.0, 1)
+    orc.run_f64(orc.ASB, J, xb, yb, 0, steps, steps, 0.5, 1.0, c0, 1.0, 1)
+    assert np.max(np.abs(xa - xb)) < 1e-11 and np.max(np.abs(ya - yb)) < 1e-11
+    assert np.max(np.abs(xa)) > 0.05  # the trajectory moved
+
+
+def test_m1_fp32_follows_fp64(orc):
+    # The FP32 restatement stays within a few ulp-scaled units of its FP64
+    # trajectory; a dropped term or wrong sign would not.
+    N, R, steps = 64, 3, 40
+    J = _sk(N, 6)
+    c0 = orc.c0_auto(orc.ASB_M1, J)
+    x32, y32 = orc.init_state("uniform", N, R, seed=4)
+    x64, y64 = x32.astype(np.float64), y32.astype(np.float64)
+    orc.run_f32(orc.ASB_M1, J, x32, y32, 0, steps, steps, 0.5, 1.0, c0, 1.0, 1)
+    orc.run_f64(orc.ASB_M1, J, x64, y64, 0, steps, steps, 0.5, 1.0, c0, 1.0, 1)
+    assert np.max(np.abs(x32 - x64)) < 1e-5 and np.max(np.abs(y32 - y64)) < 1e-5
+    # and it is its own rounding sequence, not aSB's
+    xa, ya = orc.init_state("uniform", N, R, seed=4)
+    orc.run_f32(orc.ASB, J, xa, ya, 0, steps, steps, 0.5, 1.0, c0, 1.0, 1)
+    assert not np.array_equal(ya, y32)
+
+
+def test_m1_single_step_known_values(orc):
+    # One spin, J = 0, DeltaT = 0.5 (every product with DeltaT is exact):
+    # x' = 0.05 + 0.02 * 0.5 = 0.06; y' = fl32(-x'^3 / 2 + p x' / 2 - x' / 2)
+    # + fl32(F xi / 2) + y, accumulated in FP32.
+    x1 = orc.m1_x_f32(np.float32(0.05), np.float32(0.02), 1.0, 0.5)
+    assert x1 == np.float32(np.float32(0.05) + np.float32(0.01))
+    x = np.float32(x1)
+    ny = np.float32(-(np.float64(x * x * x)) * 0.5 + np.float64(np.float32(0.25) * x) * 0.5 - np.float64(x) * 0.5)
+    ny = np.float32(ny + np.float32(np.float64(np.float32(0.3)) * 0.01565 * 0.5))
+    ny = np.float32(ny + np.float32(0.02))
+    assert orc.m1_y_f32(0.3, x, 0.02, 0.25, 1.0, 1.0, 0.5, 0.01565) == ny
+
+
 # ---- R7 energy (m2_opt.cu:192-202) ----------------------------------------------
 
 def _sym_int(N, lo, hi, rng):

@@ -146,6 +146,13 @@ int read_weight(Cursor &c, double *v)
 extern "C" SB_API int sb_edgelist_parse(const char *text, size_t len, int one_based, int32_t *N_out,
                                         float *J, int32_t ldj)
 {
+    return sb_edgelist_parse_field(text, len, one_based, N_out, J, ldj, nullptr, nullptr);
+}
+
+extern "C" SB_API int sb_edgelist_parse_field(const char *text, size_t len, int one_based, int32_t *N_out,
+                                              float *J, int32_t ldj, float *h, int32_t *has_field)
+{
+    if (has_field) *has_field = 0;
     if (!text || !N_out) return SB_E_INVALID;
     Cursor c{text, text + len};
     long long n = 0;
@@ -157,6 +164,7 @@ extern "C" SB_API int sb_edgelist_parse(const char *text, size_t len, int one_ba
     if (!J) return SB_OK;
     if (ldj < n) return SB_E_INVALID;
     for (long long i = 0; i < n; ++i) std::memset(J + (size_t)i * ldj, 0, sizeof(float) * (size_t)n);
+    if (h) std::memset(h, 0, sizeof(float) * (size_t)n);
     long long line = 1;
     for (;;) {
         long long a, b;
@@ -177,8 +185,15 @@ extern "C" SB_API int sb_edgelist_parse(const char *text, size_t len, int one_ba
             return SB_E_RANGE;
         }
         if (a == b) {
-            sbh::set_error("edge list: self-coupling i == j in triple #" + std::to_string(line - 1));
-            return SB_E_INVALID;
+            // i i w: external field h_i (the king-graph generators' convention);
+            // the reference SB rejects it (m2_opt.cu:119).
+            if (!h) {
+                sbh::set_error("edge list: self-coupling i == j in triple #" + std::to_string(line - 1));
+                return SB_E_INVALID;
+            }
+            h[a] = (float)w;
+            if (has_field) *has_field = 1;
+            continue;
         }
         J[(size_t)a * ldj + b] = (float)w;
         J[(size_t)b * ldj + a] = (float)w;
@@ -434,13 +449,20 @@ extern "C" SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *pat
     bool err = std::ferror(f) != 0;
     std::fclose(f);
     if (err) return SB_E_IO;
-    int32_t N = 0;
+    int32_t N = 0, has_field = 0;
     int rc = sb_edgelist_parse(text.data(), text.size(), one_based, &N, nullptr, 0);
     if (rc) return rc;
-    std::vector<float> J((size_t)N * N, 0.0f);
-    rc = sb_edgelist_parse(text.data(), text.size(), one_based, &N, J.data(), N);
+    std::vector<float> J((size_t)N * N, 0.0f), h((size_t)N, 0.0f);
+    rc = sb_edgelist_parse_field(text.data(), text.size(), one_based, &N, J.data(), N, h.data(), &has_field);
     if (rc) return rc;
-    return sb_problem_create(ctx, N, J.data(), N, out);
+    rc = sb_problem_create(ctx, N, J.data(), N, out);
+    if (rc || !has_field) return rc;
+    rc = sb_problem_set_field(*out, h.data());
+    if (rc) {
+        sb_problem_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
 }
 
 extern "C" SB_API int sb_problem_destroy(sb_problem *p)
@@ -452,8 +474,43 @@ extern "C" SB_API int sb_problem_destroy(sb_problem *p)
     (void)hipFree(p->J32);
     (void)hipFree(p->Jbf16);
     (void)hipFree(p->Jfp8);
+    (void)hipFree(p->h);
     p->ctx->live_problems--;
     delete p;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_set_field(sb_problem *p, const float *h)
+{
+    if (!p) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (!h) {
+        (void)hipFree(p->h);
+        p->h = nullptr;
+        p->h_integer = true;
+        return SB_OK;
+    }
+    bool integer = true;
+    for (int i = 0; i < p->N; ++i) {
+        if (!std::isfinite(h[i])) {
+            sbh::set_error("external field: non-finite h_i");
+            return SB_E_INVALID;
+        }
+        if (h[i] != std::floor(h[i])) integer = false;
+    }
+    if (!p->h) {
+        // Padded to ld with zeros like the state.
+        hipError_t e = hipMalloc(&p->h, sizeof(float) * (size_t)p->ld);
+        if (e != hipSuccess) {
+            p->h = nullptr;
+            return sbh::hip_fail(e, "hipMalloc(h)");
+        }
+    }
+    HIP_TRY(hipMemset(p->h, 0, sizeof(float) * (size_t)p->ld));
+    HIP_TRY(hipMemcpy(p->h, h, sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice));
+    p->h_integer = integer;
     return SB_OK;
 }
 
@@ -793,11 +850,14 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
 // ---------------------------------------------------------------------------
 // Solve.
 
+// Energies are exact integers (int64) when J and the external field are.
+static bool exact_energy(const sb_problem *prob) { return prob->integer && (!prob->h || prob->h_integer); }
+
 int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const float *Y, int R, sb_result *res,
                  hipStream_t s)
 {
     const int N = prob->N, ld = prob->ld;
-    HIP_TRY(launch_energy(prob->J32, ld, X, ld, N, R, ctx->part, ctx->H, s));
+    HIP_TRY(launch_energy(prob->J32, ld, prob->h, X, ld, N, R, ctx->part, ctx->H, s));
     std::vector<double> H(R);
     HIP_TRY(hipMemcpyAsync(H.data(), ctx->H, sizeof(double) * R, hipMemcpyDeviceToHost, s));
     if (res->spins) {
@@ -817,11 +877,11 @@ int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const floa
     int best = 0;
     for (int r = 0; r < R; ++r) {
         if (res->energy_f64) res->energy_f64[r] = H[r];
-        if (res->energy_i64) res->energy_i64[r] = prob->integer ? (int64_t)std::llround(H[r]) : 0;
+        if (res->energy_i64) res->energy_i64[r] = exact_energy(prob) ? (int64_t)std::llround(H[r]) : 0;
         if (H[r] < H[best]) best = r;
     }
     res->best_replica = best;
-    res->energy_exact = prob->integer ? 1 : 0;
+    res->energy_exact = exact_energy(prob) ? 1 : 0;
     return SB_OK;
 }
 
@@ -842,6 +902,7 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     if (has_prologue(p->variant)) HIP_TRY(launch_asb_local(prologue_coeffs(*p), N, R, ld, X, Y, s));
     StepArgs a{};
     a.J = p->precision == SB_PREC_ORDERED ? (const void *)prob->J32 : family_J(family, prob);
+    a.h = prob->h;
     a.ldj = ld;
     a.ldx = ld;
     a.N = N;
@@ -973,6 +1034,7 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
     a.ldx = ld;
     a.N = N;
     a.R = R;
+    a.h = prob->h;
     // aSB-m1 has the same field J.x; aSB's kick with beta = 1 returns it.
     a.c.variant = variant == SB_ASB_M1 ? SB_ASB : variant;
     a.c.beta = 1.0f;
@@ -1034,6 +1096,6 @@ extern "C" SB_API int sb_energy(sb_ctx *ctx, const float *dJ, int32_t N, int32_t
     size_t need = energy_partial_elems(N, R);
     int rc = sbh::grow(&ctx->part, &ctx->part_cap, need);
     if (rc) return rc;
-    HIP_TRY(launch_energy(dJ, ldj, dX, ldx, N, R, ctx->part, dH, s));
+    HIP_TRY(launch_energy(dJ, ldj, nullptr, dX, ldx, N, R, ctx->part, dH, s));
     return SB_OK;
 }

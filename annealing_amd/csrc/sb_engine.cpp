@@ -203,6 +203,7 @@ int Engine::rebuild()
         const TileShape sh = batch_shape_at(s.family, s.bshape);
         const int N = s.prob->N, R = s.params.replicas;
         I.J = family_J(s.family, s.prob);
+        I.h = s.prob->h;
         I.X[0] = s.X[0];
         I.X[1] = s.X[1];
         I.Y = s.Y;

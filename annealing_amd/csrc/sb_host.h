@@ -41,6 +41,8 @@ struct sb_problem {
     float *J32 = nullptr;
     uint16_t *Jbf16 = nullptr;
     uint8_t *Jfp8 = nullptr;  // only when integer and |J| <= 16
+    float *h = nullptr;       // external field [ld] (zero padded), or null
+    bool h_integer = true;
 };
 
 namespace sbh {

@@ -34,6 +34,7 @@ struct StepArgs {
     const float *X;  // [R][ldx] state read this step
     float *Xn;       // [R][ldx] state written this step
     float *Y;        // [R][ldx] updated in place
+    const float *h;  // external field [N] added to J.x (null: none)
     const void *B;   // B-operand planes of X (bkind != BK_NONE)
     void *Bn;        // planes of Xn written by the epilogue (may be null)
     size_t bplane;   // elements per plane (R * ldx)
@@ -106,7 +107,8 @@ hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X,
                        float *X2, hipStream_t s);
 hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, float *X, float *Y,
                             hipStream_t s);
-hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N, int R,
+// H_r = -sum_{i<j} J_ij s_i s_j (- sum_i h_i s_i when h is not null).
+hipError_t launch_energy(const float *J, int ldj, const float *h, const float *X, int ldx, int N, int R,
                          double *partial, double *H, hipStream_t s);
 size_t energy_partial_elems(int N, int R);
 hipError_t launch_spins(const float *X, int ldx, int N, int R, int8_t *S, hipStream_t s);
@@ -122,6 +124,7 @@ hipError_t launch_pack_fp8(const float *J, size_t n_elems, uint8_t *out, hipStre
 // while the host moves step_base (and the instance set) between replays.
 struct BatchInstance {
     const void *J;      // J in the family's format, [N][ldx]
+    const float *h;     // external field [N] or null
     float *X[2];        // ping-pong state, read X[k & 1], write X[(k + 1) & 1]
     float *Y;
     void *P[2];         // B-operand planes (split families), same ping-pong

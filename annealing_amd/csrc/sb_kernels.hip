@@ -189,6 +189,9 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
 {
     const int i = spin_base + (lane & 31);
     if (i >= a.N) return;
+    // External field: F_i = (J.x)_i + h_i, added after the sum.
+    const float hi = a.h ? a.h[i] : 0.0f;
+    auto field = [&](float f) { return a.h ? f + hi : f; };
     const int r0 = rep_base + 4 * (lane >> 5);
     const size_t o0 = (size_t)r0 * a.ldx + i;
     auto off = [&](int v) { return o0 + (size_t)(8 * (v >> 2) + (v & 3)) * a.ldx; };
@@ -202,7 +205,7 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
         }
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            sb_update(a.c, acc[v], xv[v], yv[v]);
+            sb_update(a.c, field(acc[v]), xv[v], yv[v]);
             a.Xn[off(v)] = xv[v];
             a.Y[off(v)] = yv[v];
         }
@@ -216,7 +219,7 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
     for (int v = 0; v < 16; ++v) {
         if (r0 + 8 * (v >> 2) + (v & 3) >= a.R) continue;
         float x = a.X[off(v)], y = a.Y[off(v)];
-        sb_update(a.c, acc[v], x, y);
+        sb_update(a.c, field(acc[v]), x, y);
         a.Xn[off(v)] = x;
         a.Y[off(v)] = y;
         if (a.Bn) write_plane1(a.bkind, x, a.Bn, a.bplane, off(v));
@@ -840,6 +843,7 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
 #pragma unroll
         for (int rr = 0; rr < kThinMaxR; ++rr)
             if (qq == q && rr == r) F = acc[qq][rr];
+    if (a.h) F = F + a.h[i];  // external field, after the sum
     const size_t o = (size_t)r * a.ldx + i;
     float x = a.X[o], y = a.Y[o];
     sb_update(a.c, F, x, y);
@@ -897,6 +901,7 @@ __global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
         if (sgn) v = sgn1(v);
         F = __builtin_fmaf(Jr[j], v, F);
     }
+    if (a.h) F = F + a.h[i];  // external field, after the sum
     float x = xr[i], y = a.Y[(size_t)r * a.ldx + i];
     sb_update(a.c, F, x, y);
     a.Xn[(size_t)r * a.ldx + i] = x;
@@ -1146,6 +1151,7 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         if (!I.active || k < 0 || k >= I.steps) continue;
         StepArgs a;
         a.J = I.J;
+        a.h = I.h;
         a.X = I.X[k & 1];
         a.Xn = I.X[(k + 1) & 1];
         a.Y = I.Y;
@@ -1504,13 +1510,30 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *part, int ti
     H[r] = -0.5 * s;
 }
 
+// The field term -sum_i h_i s_i per replica: one workgroup per replica, a
+// fixed per-thread order and a fixed tree (deterministic; exact for integer h).
+__global__ __launch_bounds__(256) void k_energy_field(const float *h, const float *X, int ldx, int N, double *H)
+{
+    __shared__ double red[256];
+    const int r = blockIdx.x, t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < N; i += 256) s += X[(size_t)r * ldx + i] > 0.0f ? (double)h[i] : -(double)h[i];
+    red[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) H[r] -= red[0];
+}
+
 size_t energy_partial_elems(int N, int R)
 {
     return (size_t)((N + EN_T - 1) / EN_T) * (size_t)R;
 }
 
-hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N, int R, double *partial,
-                         double *H, hipStream_t s)
+hipError_t launch_energy(const float *J, int ldj, const float *h, const float *X, int ldx, int N, int R,
+                         double *partial, double *H, hipStream_t s)
 {
     if (N <= 0 || R <= 0) return hipSuccess;
     int tiles_m = (N + EN_T - 1) / EN_T, tiles_n = (R + EN_T - 1) / EN_T;
@@ -1519,6 +1542,9 @@ hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_energy_final, dim3((R + 255) / 256), dim3(256), 0, s, partial, tiles_m, R, H);
+    e = hipGetLastError();
+    if (e != hipSuccess || !h) return e;
+    hipLaunchKernelGGL(k_energy_field, dim3(R), dim3(256), 0, s, h, X, ldx, N, H);
     return hipGetLastError();
 }
 

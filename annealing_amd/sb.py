@@ -32,10 +32,10 @@ INITS = {"uniform": INIT_UNIFORM, "reference": INIT_REFERENCE}
 # Every symbol include/sb.h declares.
 EXPORTS = (
     "sb_version", "sb_status_string", "sb_last_error_message", "sb_default_params",
-    "sb_edgelist_parse", "sb_instance_stats", "sb_c0_auto", "sb_generate",
+    "sb_edgelist_parse", "sb_edgelist_parse_field", "sb_instance_stats", "sb_c0_auto", "sb_generate",
     "sb_ctx_create", "sb_ctx_destroy", "sb_problem_create",
     "sb_problem_from_edgelist_file", "sb_problem_generate", "sb_problem_read_j",
-    "sb_problem_destroy", "sb_problem_info",
+    "sb_problem_destroy", "sb_problem_info", "sb_problem_set_field",
     "sb_solve", "sb_step_coeffs_for", "sb_step", "sb_field", "sb_init_state", "sb_asb_prologue",
     "sb_energy", "sb_batch_solve", "sb_server_create", "sb_server_submit",
     "sb_server_submit_generated", "sb_server_wait", "sb_server_cancel", "sb_server_stats_get",
@@ -116,6 +116,9 @@ def _bind(lib: C.CDLL) -> None:
         "sb_last_error_message": (C.c_char_p, []),
         "sb_default_params": (C.c_int, [C.POINTER(Params)]),
         "sb_edgelist_parse": (C.c_int, [C.c_char_p, SZ, C.c_int, C.POINTER(I32), P, I32]),
+        "sb_edgelist_parse_field": (C.c_int, [C.c_char_p, SZ, C.c_int, C.POINTER(I32), P, I32, P,
+                                              C.POINTER(I32)]),
+        "sb_problem_set_field": (C.c_int, [P, P]),
         "sb_instance_stats": (C.c_int, [I32, P, I32, C.POINTER(D), C.POINTER(I32), C.POINTER(D)]),
         "sb_c0_auto": (D, [I32, I32, D, D]),
         "sb_generate": (C.c_int, [I32, I32, D, U64, P]),
@@ -182,6 +185,20 @@ def parse_edgelist(text: str | bytes, one_based: bool = False) -> np.ndarray:
     J = np.zeros((n.value, n.value), dtype=np.float32)
     _check(lib.sb_edgelist_parse(b, len(b), int(one_based), C.byref(n), _ptr(J), n.value), "sb_edgelist_parse")
     return J
+
+
+def parse_edgelist_field(text: str | bytes, one_based: bool = False):
+    """(J, h) from an edge list whose "i i w" lines are an external field h_i;
+    h is None when the text has none."""
+    lib = load()
+    b = text.encode() if isinstance(text, str) else text
+    n, has = C.c_int32(0), C.c_int32(0)
+    _check(lib.sb_edgelist_parse(b, len(b), int(one_based), C.byref(n), None, 0), "sb_edgelist_parse")
+    J = np.zeros((n.value, n.value), dtype=np.float32)
+    h = np.zeros(n.value, dtype=np.float32)
+    _check(lib.sb_edgelist_parse_field(b, len(b), int(one_based), C.byref(n), _ptr(J), n.value, _ptr(h),
+                                       C.byref(has)), "sb_edgelist_parse_field")
+    return J, (h if has.value else None)
 
 
 def instance_stats(J: np.ndarray):
@@ -283,6 +300,14 @@ class Problem:
         self.sigma = sigma.value
         self.split_jfmt = jf.value
 
+    def set_field(self, h: np.ndarray | None) -> None:
+        if h is None:
+            _check(load().sb_problem_set_field(self.handle, None), "sb_problem_set_field")
+            return
+        h = np.ascontiguousarray(h, dtype=np.float32)
+        assert h.shape == (self.N,)
+        _check(load().sb_problem_set_field(self.handle, _ptr(h)), "sb_problem_set_field")
+
     def read_j(self) -> np.ndarray:
         J = np.empty((self.N, self.N), dtype=np.float32)
         _check(load().sb_problem_read_j(self.handle, _ptr(J), self.N), "sb_problem_read_j")
@@ -330,12 +355,16 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def problem(self, J: np.ndarray) -> Problem:
+    def problem(self, J: np.ndarray, h: np.ndarray | None = None) -> Problem:
+        """Upload J (and an optional external field h, N floats)."""
         J = np.ascontiguousarray(J, dtype=np.float32)
-        h = C.c_void_p()
-        _check(self.lib.sb_problem_create(self.handle, J.shape[0], _ptr(J), J.shape[1], C.byref(h)),
+        ph = C.c_void_p()
+        _check(self.lib.sb_problem_create(self.handle, J.shape[0], _ptr(J), J.shape[1], C.byref(ph)),
                "sb_problem_create")
-        return Problem(self, h)
+        prob = Problem(self, ph)
+        if h is not None:
+            prob.set_field(h)
+        return prob
 
     def generate(self, kind: int, N: int, p: float = 1.0, seed: int = 0) -> Problem:
         """Synthetic instance generated on the device (see sb_generate)."""

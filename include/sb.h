@@ -145,6 +145,12 @@ SB_API int sb_default_params(sb_params *p);
  * read N only.  J is written densely as [N][ldj].  Weights may be real. */
 SB_API int sb_edgelist_parse(const char *text, size_t len, int one_based, int32_t *N,
                              float *J, int32_t ldj);
+/* sb_edgelist_parse that also accepts "i i w" lines as an external field
+ * h_i (the king-graph generators' convention; the reference SB rejects them,
+ * m2_opt.cu:119).  h holds N floats (zeroed first); *has_field is set when
+ * any such line was read. */
+SB_API int sb_edgelist_parse_field(const char *text, size_t len, int one_based, int32_t *N,
+                                   float *J, int32_t ldj, float *h, int32_t *has_field);
 /* sigma_J, integer flag and max |J_ij| of a dense symmetric J. */
 SB_API int sb_instance_stats(int32_t N, const float *J, int32_t ldj, double *sigma,
                              int32_t *is_integer, double *max_abs);
@@ -166,8 +172,15 @@ SB_API int sb_ctx_destroy(sb_ctx *ctx);
  * device, and an FP8 E4M3 form when exact (integer |J| <= 16). */
 SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, int32_t ldj,
                              sb_problem **out);
+/* Reads the file with sb_edgelist_parse_field; "i i w" lines set the field. */
 SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *path, int one_based,
                                          sb_problem **out);
+/* External field h (N host floats; NULL removes it).  The force on spin i
+ * becomes (J.x)_i + h_i (J.sign(x) + h for dSB), added after the coupling
+ * sum, and energies are H = -sum_{i<j} J_ij s_i s_j - sum_i h_i s_i (exact
+ * int64 when J and h are integers).  sb_step and sb_energy on caller buffers
+ * take no field. */
+SB_API int sb_problem_set_field(sb_problem *prob, const float *h);
 /* Generate a synthetic instance on the device (kinds as sb_generate; the
  * same J as sb_generate gives on the host, up to rare last-bit differences of
  * libm in the SK kinds). */
@@ -209,10 +222,11 @@ SB_API int sb_step(sb_ctx *ctx, int32_t precision, const void *dJ, int32_t jfmt,
                    int32_t R, int32_t ldx, const sb_step_coeffs *c, void *stream);
 
 /* Local field of given amplitudes through the product kernels: F = J.X
- * (variant SB_BSB/SB_ASB) or J.sign(X) (SB_DSB), host X and F [R][N], with the
- * arithmetic of `precision` (FP32 MFMA, BF16 split, ordered).  The
- * reference's field alone (cublasSgemv, m2_opt.cu:169); used to report field
- * errors against an FP64 trajectory. */
+ * (variant SB_BSB/SB_ASB/SB_ASB_M1) or J.sign(X) (SB_DSB), plus h when the
+ * problem has an external field; host X and F [R][N], with the arithmetic of
+ * `precision` (FP32 MFMA, BF16 split, ordered; R <= 4 uses the thin kernels).
+ * The reference's field alone (cublasSgemv, m2_opt.cu:169); used to report
+ * field errors against an FP64 trajectory. */
 SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision, int32_t variant, const float *X,
                     int32_t R, float *F);
 

@@ -43,7 +43,11 @@ def load():
             "orc_m1_y_f32": (F, [F, F, F, F, D, D, D, D]),
             "orc_run_f32": (I, [I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
             "orc_run_f64": (I, [I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
-            "orc_run_f32_order": (I, [I, I, I, I, P, I, P, P, I, I, I, I, D, D, D, D, I, I]),
+            "orc_run_f32_order": (I, [I, I, I, I, P, I, P, P, P, I, I, I, I, D, D, D, D, I, I]),
+            "orc_run_f64_h": (I, [I, I, I, P, I, P, P, P, I, I, I, I, D, D, D, D, I, I]),
+            "orc_energy_field_i64": (I, [I, I, P, P, P]),
+            "orc_energy_field_f64": (None, [I, I, P, P, P]),
+            "orc_edgelist_fill_h": (I, [C.c_char_p, C.c_size_t, I, I, P, I, P]),
             "orc_spins_f32": (None, [I, I, P, I, P]),
             "orc_energy_i64": (I, [I, I, P, I, P, P]),
             "orc_energy_f64": (None, [I, I, P, I, P, P]),
@@ -151,24 +155,34 @@ def m1_y_f32(F, x, y, p, a0, K, dt, xi):
     return load().orc_m1_y_f32(F, x, y, p, a0, K, dt, xi)
 
 
+def _hp(h, N):
+    if h is None:
+        return None, None
+    h = np.ascontiguousarray(h, np.float32)
+    assert h.shape == (N,)
+    return h, _p(h)
+
+
 def run_f32(variant, J, x, y, k0, k1, steps, dt=1.0, a0=1.0, c0=0.0, K=1.0, M=2, nthreads=1,
-            order=ORDER_SEQ):
+            order=ORDER_SEQ, h=None):
     """Advance float32 state (x, y) [R, ld] in place over steps [k0, k1)."""
     J = np.ascontiguousarray(J, np.float32)
     assert x.dtype == np.float32 and y.dtype == np.float32 and x.flags.c_contiguous
     R, ld = x.shape
-    rc = load().orc_run_f32_order(order, variant, J.shape[0], R, _p(J), J.shape[1], _p(x), _p(y), ld, k0,
-                                  k1, steps, dt, a0, c0, K, M, nthreads)
+    hk, hptr = _hp(h, J.shape[0])
+    rc = load().orc_run_f32_order(order, variant, J.shape[0], R, _p(J), J.shape[1], hptr, _p(x), _p(y), ld,
+                                  k0, k1, steps, dt, a0, c0, K, M, nthreads)
     if rc:
         raise RuntimeError(f"orc_run_f32 failed: {rc}")
 
 
-def run_f64(variant, J, x, y, k0, k1, steps, dt=1.0, a0=1.0, c0=0.0, K=1.0, M=2, nthreads=1):
+def run_f64(variant, J, x, y, k0, k1, steps, dt=1.0, a0=1.0, c0=0.0, K=1.0, M=2, nthreads=1, h=None):
     J = np.ascontiguousarray(J, np.float32)
     assert x.dtype == np.float64 and y.dtype == np.float64 and x.flags.c_contiguous
     R, ld = x.shape
-    rc = load().orc_run_f64(variant, J.shape[0], R, _p(J), J.shape[1], _p(x), _p(y), ld, k0, k1, steps,
-                            dt, a0, c0, K, M, nthreads)
+    hk, hptr = _hp(h, J.shape[0])
+    rc = load().orc_run_f64_h(variant, J.shape[0], R, _p(J), J.shape[1], hptr, _p(x), _p(y), ld, k0, k1,
+                              steps, dt, a0, c0, K, M, nthreads)
     if rc:
         raise RuntimeError(f"orc_run_f64 failed: {rc}")
 
@@ -181,7 +195,7 @@ def spins(x, N):
     return s
 
 
-def energy_i64(J, s):
+def energy_i64(J, s, h=None):
     J = np.ascontiguousarray(J, np.float32)
     s = np.ascontiguousarray(s, np.int8)
     R, N = s.shape
@@ -189,15 +203,22 @@ def energy_i64(J, s):
     rc = load().orc_energy_i64(N, R, _p(J), J.shape[1], _p(s), _p(H))
     if rc:
         raise ValueError("J is not integer-valued")
+    if h is not None:
+        hk, hptr = _hp(h, N)
+        if load().orc_energy_field_i64(N, R, hptr, _p(s), _p(H)):
+            raise ValueError("h is not integer-valued")
     return H
 
 
-def energy_f64(J, s):
+def energy_f64(J, s, h=None):
     J = np.ascontiguousarray(J, np.float32)
     s = np.ascontiguousarray(s, np.int8)
     R, N = s.shape
     H = np.zeros(R, np.float64)
     load().orc_energy_f64(N, R, _p(J), J.shape[1], _p(s), _p(H))
+    if h is not None:
+        hk, hptr = _hp(h, N)
+        load().orc_energy_field_f64(N, R, hptr, _p(s), _p(H))
     return H
 
 
@@ -223,12 +244,25 @@ def parse_edgelist(text, one_based=False):
     return rc, (J if rc == 0 else None)
 
 
+def parse_edgelist_field(text, one_based=False):
+    """(rc, J, h): "i i w" lines read as the external field h."""
+    b = text.encode() if isinstance(text, str) else text
+    n = C.c_int()
+    rc = load().orc_edgelist_header(b, len(b), C.byref(n))
+    if rc:
+        return rc, None, None
+    J = np.zeros((n.value, n.value), np.float32)
+    h = np.zeros(n.value, np.float32)
+    rc = load().orc_edgelist_fill_h(b, len(b), int(one_based), n.value, _p(J), n.value, _p(h))
+    return rc, (J if rc == 0 else None), (h if rc == 0 else None)
+
+
 def solve_f32(variant, J, R, steps, init="uniform", seed=0, dt=1.0, a0=1.0, c0=None, K=1.0, M=2,
-              nthreads=1, order=ORDER_SEQ):
+              nthreads=1, order=ORDER_SEQ, h=None):
     """Whole solve on the CPU: init, (aSB prologue), steps; returns x, y [R, ld]."""
     N = J.shape[0]
     if c0 is None or c0 <= 0:
         c0 = c0_auto(variant, J, a0)
     x, y = init_state(init, N, R, seed)
-    run_f32(variant, J, x, y, 0, steps, steps, dt, a0, c0, K, M, nthreads, order)
+    run_f32(variant, J, x, y, 0, steps, steps, dt, a0, c0, K, M, nthreads, order, h)
     return x, y, c0

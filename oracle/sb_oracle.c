@@ -381,8 +381,15 @@ static void field_order(int order, int variant, int N, const float *J, int ldj, 
     field_rows_f32_lanes(N, J, ldj, s, F);
 }
 
-static int run_f32_replica(int order, int variant, int N, const float *J, int ldj, float *x,
-                           float *y, float *F, float *s, int k0, int k1, int steps,
+/* External field: F_i = (J.x)_i + h_i, one FP32 addition after the sum. */
+static void add_field(int N, const float *h, float *F)
+{
+    if (h)
+        for (int i = 0; i < N; ++i) F[i] = F[i] + h[i];
+}
+
+static int run_f32_replica(int order, int variant, int N, const float *J, int ldj, const float *h,
+                           float *x, float *y, float *F, float *s, int k0, int k1, int steps,
                            double dt, double a0, double c0, double K, int M)
 {
     double dts = dt / (double)M;
@@ -395,14 +402,17 @@ static int run_f32_replica(int order, int variant, int N, const float *J, int ld
             const float p = (float)k / (float)steps;
             for (int i = 0; i < N; ++i) x[i] = orc_m1_x_f32(x[i], y[i], a0, dt);
             field_order(order, ORC_BSB, N, J, ldj, x, s, F);
+            add_field(N, h, F);
             for (int i = 0; i < N; ++i) y[i] = orc_m1_y_f32(F[i], x[i], y[i], p, a0, K, dt, c0);
         } else if (variant == ORC_ASB) {
             /* m2_opt.cu:168-169: local sub-steps, then the kick with new x */
             for (int i = 0; i < N; ++i) orc_asb_local_f32(&x[i], &y[i], cf[3], a0, K, dts, M);
             field_order(order, variant, N, J, ldj, x, s, F);
+            add_field(N, h, F);
             for (int i = 0; i < N; ++i) y[i] = fmaf(cf[1], F[i], y[i]);
         } else {
             field_order(order, variant, N, J, ldj, x, s, F);
+            add_field(N, h, F);
             for (int i = 0; i < N; ++i) orc_bsb_update_f32(F[i], cf[0], cf[1], cf[2], &x[i], &y[i]);
         }
     }
@@ -423,12 +433,12 @@ int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
                 float *y, int ld, int k0, int k1, int steps, double dt, double a0,
                 double c0, double K, int M, int nthreads)
 {
-    return orc_run_f32_order(ORC_ORDER_SEQ, variant, N, R, J, ldj, x, y, ld, k0, k1, steps, dt,
-                             a0, c0, K, M, nthreads);
+    return orc_run_f32_order(ORC_ORDER_SEQ, variant, N, R, J, ldj, NULL, x, y, ld, k0, k1, steps,
+                             dt, a0, c0, K, M, nthreads);
 }
 
-int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int ldj, float *x,
-                      float *y, int ld, int k0, int k1, int steps, double dt, double a0,
+int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int ldj, const float *h,
+                      float *x, float *y, int ld, int k0, int k1, int steps, double dt, double a0,
                       double c0, double K, int M, int nthreads)
 {
     int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
@@ -446,7 +456,7 @@ int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int 
 #pragma omp for schedule(static)
 #endif
         for (int r = 0; r < R; ++r)
-            run_f32_replica(order, variant, N, J, ldj, x + (size_t)r * ld, y + (size_t)r * ld,
+            run_f32_replica(order, variant, N, J, ldj, h, x + (size_t)r * ld, y + (size_t)r * ld,
                             F, s, k0, k1, steps, dt, a0, c0, K, M);
         free(F);
         free(s);
@@ -457,6 +467,14 @@ int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int 
 int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
                 double *y, int ld, int k0, int k1, int steps, double dt, double a0,
                 double c0, double K, int M, int nthreads)
+{
+    return orc_run_f64_h(variant, N, R, J, ldj, NULL, x, y, ld, k0, k1, steps, dt, a0, c0, K, M,
+                         nthreads);
+}
+
+int orc_run_f64_h(int variant, int N, int R, const float *J, int ldj, const float *h, double *x,
+                  double *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                  double c0, double K, int M, int nthreads)
 {
     int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
     if (rc) return rc;
@@ -478,6 +496,8 @@ int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
                     /* the same step in FP64 throughout */
                     for (int i = 0; i < N; ++i) xr[i] = xr[i] + a0 * yr[i] * dt;
                     orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
+                    if (h)
+                        for (int i = 0; i < N; ++i) F[i] += (double)h[i];
                     for (int i = 0; i < N; ++i) {
                         const double xi_ = xr[i];
                         yr[i] = (-K * xi_ * xi_ * xi_ * dt + p * xi_ * dt - a0 * xi_ * dt) + F[i] * c0 * dt + yr[i];
@@ -485,9 +505,13 @@ int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
                 } else if (variant == ORC_ASB) {
                     for (int i = 0; i < N; ++i) asb_local_f64(&xr[i], &yr[i], p, a0, K, dts, M);
                     orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
+                    if (h)
+                        for (int i = 0; i < N; ++i) F[i] += (double)h[i];
                     for (int i = 0; i < N; ++i) yr[i] = fma(dt * c0, F[i], yr[i]);
                 } else {
                     orc_field_f64(N, 1, J, ldj, xr, ld, variant == ORC_DSB, F);
+                    if (h)
+                        for (int i = 0; i < N; ++i) F[i] += (double)h[i];
                     double alpha = dt * (a0 - a0 * p), beta = dt * c0, gamma = dt * a0;
                     for (int i = 0; i < N; ++i) bsb_update_f64(F[i], alpha, beta, gamma, &xr[i], &yr[i]);
                 }
@@ -535,6 +559,29 @@ void orc_energy_f64(int N, int R, const float *J, int ldj, const int8_t *s, doub
             for (int j = i + 1; j < N; ++j)
                 E += (double)(sr[i] * sr[j]) * (double)J[(size_t)i * ldj + j];
         H[r] = -E;
+    }
+}
+
+/* The external-field term -sum_i h_i s_i added to H (int64 when h is
+ * integer-valued; returns ORC_E_INVALID otherwise). */
+int orc_energy_field_i64(int N, int R, const float *h, const int8_t *s, int64_t *H)
+{
+    for (int i = 0; i < N; ++i)
+        if (h[i] != floorf(h[i])) return ORC_E_INVALID;
+    for (int r = 0; r < R; ++r) {
+        int64_t E = 0;
+        for (int i = 0; i < N; ++i) E += (int64_t)s[(size_t)r * N + i] * (int64_t)h[i];
+        H[r] -= E;
+    }
+    return ORC_OK;
+}
+
+void orc_energy_field_f64(int N, int R, const float *h, const int8_t *s, double *H)
+{
+    for (int r = 0; r < R; ++r) {
+        double E = 0.0;
+        for (int i = 0; i < N; ++i) E += (double)s[(size_t)r * N + i] * (double)h[i];
+        H[r] -= E;
     }
 }
 
@@ -617,6 +664,13 @@ int orc_edgelist_header(const char *text, size_t len, int *N)
 int orc_edgelist_fill(const char *text, size_t len, int one_based, int N, float *J,
                       int ldj)
 {
+    return orc_edgelist_fill_h(text, len, one_based, N, J, ldj, NULL);
+}
+
+int orc_edgelist_fill_h(const char *text, size_t len, int one_based, int N, float *J,
+                        int ldj, float *h)
+{
+    if (h) memset(h, 0, sizeof(float) * (size_t)N);
     cursor c = {text, text + len};
     long n;
     if (read_long(&c, &n) != 1 || n != N) return ORC_E_PARSE;
@@ -632,7 +686,11 @@ int orc_edgelist_fill(const char *text, size_t len, int one_based, int N, float 
             b -= 1;
         }
         if (a < 0 || b < 0 || a >= N || b >= N) return ORC_E_RANGE;
-        if (a == b) return ORC_E_INVALID;
+        if (a == b) {
+            if (!h) return ORC_E_INVALID;
+            h[a] = (float)w; /* "i i w": external field */
+            continue;
+        }
         J[(size_t)a * ldj + b] = (float)w;
         J[(size_t)b * ldj + a] = (float)w;
     }

@@ -115,15 +115,23 @@ int orc_run_f32(int variant, int N, int R, const float *J, int ldj, float *x,
                 float *y, int ld, int k0, int k1, int steps, double dt, double a0,
                 double c0, double K, int M, int nthreads);
 
-/* orc_run_f32 with the field in the given order (ORC_ORDER_SEQ or _LANES). */
+/* orc_run_f32 with the field in the given order (ORC_ORDER_SEQ or _LANES) and
+ * an optional external field h [N] (NULL: none), added to each F_i after the
+ * sum: F_i = fl32((J.x)_i + h_i). */
 int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int ldj,
-                      float *x, float *y, int ld, int k0, int k1, int steps, double dt,
-                      double a0, double c0, double K, int M, int nthreads);
+                      const float *h, float *x, float *y, int ld, int k0, int k1, int steps,
+                      double dt, double a0, double c0, double K, int M, int nthreads);
 
-/* Same schedule in FP64 (the FP64 oracle trajectory). */
+/* Same schedule in FP64 (the FP64 oracle trajectory); _h with a field. */
 int orc_run_f64(int variant, int N, int R, const float *J, int ldj, double *x,
                 double *y, int ld, int k0, int k1, int steps, double dt, double a0,
                 double c0, double K, int M, int nthreads);
+int orc_run_f64_h(int variant, int N, int R, const float *J, int ldj, const float *h, double *x,
+                  double *y, int ld, int k0, int k1, int steps, double dt, double a0,
+                  double c0, double K, int M, int nthreads);
+/* -sum_i h_i s_i added to H[r] (the external-field part of the energy). */
+int orc_energy_field_i64(int N, int R, const float *h, const int8_t *s, int64_t *H);
+void orc_energy_field_f64(int N, int R, const float *h, const int8_t *s, double *H);
 
 /* Sign readout s_i = x_i > 0 ? +1 : -1 (m2_opt.cu:196-197). */
 void orc_spins_f32(int N, int R, const float *x, int ld, int8_t *s);
@@ -147,6 +155,9 @@ int orc_ground_state(int N, const float *J, int ldj, double *H, int8_t *s);
 int orc_edgelist_header(const char *text, size_t len, int *N);
 int orc_edgelist_fill(const char *text, size_t len, int one_based, int N,
                       float *J, int ldj);
+/* As orc_edgelist_fill, with "i i w" lines read into h [N] (zeroed first). */
+int orc_edgelist_fill_h(const char *text, size_t len, int one_based, int N, float *J,
+                        int ldj, float *h);
 
 /* 1 when the host CPU has FMA (the FP32 restatement needs exact fmaf). */
 int orc_has_fma(void);

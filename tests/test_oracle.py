@@ -273,6 +273,71 @@ def test_m1_single_step_known_values(orc):
     assert orc.m1_y_f32(0.3, x, 0.02, 0.25, 1.0, 1.0, 0.5, 0.01565) == ny
 
 
+# ---- external field h (next-step row: "i i w" lines) -----------------------------
+
+def _augment(J, h):
+    """The field-free model on N + 1 spins with J[i][N] = J[N][i] = h_i."""
+    N = J.shape[0]
+    Ja = np.zeros((N + 1, N + 1), np.float32)
+    Ja[:N, :N] = J
+    Ja[:N, N] = h
+    Ja[N, :N] = h
+    return Ja
+
+
+def test_field_energy_equals_auxiliary_spin(orc):
+    # H(s; J, h) = -sum_{i<j} J s s - sum h s is the field-free energy of the
+    # augmented model with the extra spin at +1; its ground state (a global
+    # minimum up to the all-flip symmetry) is the field model's.
+    rng = np.random.default_rng(21)
+    N = 11
+    J = rng.integers(-3, 4, (N, N)).astype(np.float32)
+    J = np.triu(J, 1)
+    J = (J + J.T).astype(np.float32)
+    h = rng.integers(-4, 5, N).astype(np.float32)
+    Ja = _augment(J, h)
+    S = np.where(rng.random((40, N)) < 0.5, 1, -1).astype(np.int8)
+    Sa = np.concatenate([S, np.ones((40, 1), np.int8)], axis=1)
+    assert np.array_equal(orc.energy_i64(J, S, h), orc.energy_i64(Ja, Sa))
+    assert np.allclose(orc.energy_f64(J, S, h), orc.energy_f64(Ja, Sa), rtol=0, atol=1e-9)
+    Hg, sg = orc.ground_state(Ja)
+    sg = sg if sg[N] == 1 else -sg
+    assert orc.energy_i64(J, sg[None, :N], h)[0] == Hg
+    # J = 0: the ground energy is -sum |h|
+    assert orc.ground_state(_augment(np.zeros((N, N), np.float32), h))[0] == -np.abs(h).sum()
+
+
+def test_field_step_equals_auxiliary_spin_step(orc):
+    # One step: the field F_i = fl32((J.x)_i + h_i) equals the augmented
+    # sequential sum whose last term is h_i * 1, bit for bit (SEQ order).  bSB
+    # and dSB take the field on the incoming state, where the extra spin is
+    # still +1 (aSB moves x before its field).
+    N, R = 90, 3
+    rng = np.random.default_rng(22)
+    J = (rng.standard_normal((N, N)) / np.sqrt(N)).astype(np.float32)
+    J = np.triu(J, 1)
+    J = (J + J.T).astype(np.float32)
+    h = (0.3 * rng.standard_normal(N)).astype(np.float32)
+    for variant in (orc.BSB, orc.DSB):
+        x, y = orc.init_state("uniform", N, R, seed=2)
+        xa, ya = orc.init_state("uniform", N + 1, R, seed=2)
+        xa[:, :N], ya[:, :N] = x[:, :N], y[:, :N]
+        xa[:, N] = 1.0
+        ya[:, N] = 0.0
+        orc.run_f32(variant, J, x, y, 0, 1, 50, 0.5, 1.0, 0.2, 1.0, 2, h=h)
+        orc.run_f32(variant, _augment(J, h), xa, ya, 0, 1, 50, 0.5, 1.0, 0.2, 1.0, 2)
+        assert np.array_equal(x[:, :N], xa[:, :N]) and np.array_equal(y[:, :N], ya[:, :N]), variant
+
+
+def test_edgelist_field_lines(orc):
+    text = "4\n0 1 1\n2 2 -3\n1 3 -1\n0 0 2\n"
+    rc, J, h = orc.parse_edgelist_field(text)
+    assert rc == 0 and list(h) == [2.0, 0.0, -3.0, 0.0]
+    assert J[0, 1] == 1 and J[1, 3] == -1 and J[2, 2] == 0
+    rc, _ = orc.parse_edgelist(text)
+    assert rc != 0  # without a field output, "i i w" stays an error
+
+
 # ---- R7 energy (m2_opt.cu:192-202) ----------------------------------------------
 
 def _sym_int(N, lo, hi, rng):

@@ -304,6 +304,11 @@ static void free_ws(sb_ctx *c)
     (void)hipFree(c->tmp);
     (void)hipFree(c->P0);
     (void)hipFree(c->P1);
+    (void)hipFree(c->XT0);
+    (void)hipFree(c->XT1);
+    (void)hipFree(c->YT);
+    c->XT0 = c->XT1 = c->YT = nullptr;
+    c->t_cap = 0;
     c->P0 = c->P1 = nullptr;
     c->plane_cap = 0;
     c->X0 = c->X1 = c->Y = nullptr;
@@ -475,6 +480,9 @@ extern "C" SB_API int sb_problem_destroy(sb_problem *p)
     (void)hipFree(p->Jbf16);
     (void)hipFree(p->Jfp8);
     (void)hipFree(p->h);
+    (void)hipFree(p->csr_ptr);
+    (void)hipFree(p->csr_col);
+    (void)hipFree(p->csr_w);
     p->ctx->live_problems--;
     delete p;
     return SB_OK;
@@ -511,6 +519,71 @@ extern "C" SB_API int sb_problem_set_field(sb_problem *p, const float *h)
     HIP_TRY(hipMemset(p->h, 0, sizeof(float) * (size_t)p->ld));
     HIP_TRY(hipMemcpy(p->h, h, sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice));
     p->h_integer = integer;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_set_sparse(sb_problem *p, int32_t enable)
+{
+    if (!p) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    (void)hipFree(p->csr_ptr);
+    (void)hipFree(p->csr_col);
+    (void)hipFree(p->csr_w);
+    p->csr_ptr = p->csr_col = nullptr;
+    p->csr_w = nullptr;
+    p->nnz = 0;
+    if (!enable) return SB_OK;
+    // CSR of the FP32 J, rows in ascending column order.
+    const int N = p->N;
+    std::vector<float> J((size_t)N * N);
+    HIP_TRY(hipMemcpy2D(J.data(), sizeof(float) * (size_t)N, p->J32, sizeof(float) * (size_t)p->ld,
+                        sizeof(float) * (size_t)N, (size_t)N, hipMemcpyDeviceToHost));
+    std::vector<int> ptr((size_t)N + 1, 0);
+    long long nnz = 0;
+    for (int i = 0; i < N; ++i) {
+        for (int j = 0; j < N; ++j) nnz += J[(size_t)i * N + j] != 0.0f;
+        if (nnz > INT32_MAX) {
+            sbh::set_error("sparse J: more than 2^31 - 1 nonzeros");
+            return SB_E_INVALID;
+        }
+        ptr[(size_t)i + 1] = (int)nnz;
+    }
+    std::vector<int> col((size_t)std::max(nnz, 1ll));
+    std::vector<float> w((size_t)std::max(nnz, 1ll));
+    size_t k = 0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            const float v = J[(size_t)i * N + j];
+            if (v != 0.0f) {
+                col[k] = j;
+                w[k] = v;
+                ++k;
+            }
+        }
+    hipError_t e = hipMalloc(&p->csr_ptr, sizeof(int) * ptr.size());
+    if (e == hipSuccess) e = hipMalloc(&p->csr_col, sizeof(int) * col.size());
+    if (e == hipSuccess) e = hipMalloc(&p->csr_w, sizeof(float) * w.size());
+    if (e == hipSuccess) e = hipMemcpy(p->csr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->csr_col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->csr_w, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p->csr_ptr);
+        (void)hipFree(p->csr_col);
+        (void)hipFree(p->csr_w);
+        p->csr_ptr = p->csr_col = nullptr;
+        p->csr_w = nullptr;
+        return sbh::hip_fail(e, "sparse J upload");
+    }
+    p->nnz = nnz;
+    return SB_OK;
+}
+
+extern "C" SB_API int sb_problem_sparse_nnz(const sb_problem *p, int64_t *nnz)
+{
+    if (!p || !nnz) return SB_E_INVALID;
+    *nnz = p->nnz;
     return SB_OK;
 }
 
@@ -885,8 +958,77 @@ int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const floa
     return SB_OK;
 }
 
+int ensure_sparse_ws(sb_ctx *ctx, size_t elems)
+{
+    if (ctx->t_cap >= elems) return SB_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    size_t c0 = ctx->t_cap, c1 = ctx->t_cap, c2 = ctx->t_cap;
+    int rc;
+    if ((rc = grow(&ctx->XT0, &c0, elems))) return rc;
+    if ((rc = grow(&ctx->XT1, &c1, elems))) return rc;
+    if ((rc = grow(&ctx->YT, &c2, elems))) return rc;
+    ctx->t_cap = elems;
+    return SB_OK;
+}
+
+SparseArgs sparse_args(const sb_problem *prob, int R, int ldr)
+{
+    SparseArgs a{};
+    a.ptr = prob->csr_ptr;
+    a.col = prob->csr_col;
+    a.w = prob->csr_w;
+    a.h = prob->h;
+    a.N = prob->N;
+    a.R = R;
+    a.ldr = ldr;
+    return a;
+}
+
+// Solve on the sparse family: init and readout in the replica-major workspace,
+// the step loop on the spin-major copy (padding replicas zero).
+static int solve_sparse(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)
+{
+    const int N = prob->N, ld = prob->ld, R = p->replicas, ldr = round_up(R, 64);
+    hipStream_t s = ctx->stream;
+    int rc = ensure_workspace(ctx, (size_t)R * ld, R, N);
+    if (rc) return rc;
+    if ((rc = ensure_sparse_ws(ctx, (size_t)N * ldr))) return rc;
+    float *X = ctx->X0, *Y = ctx->Y;
+    HIP_TRY(launch_init(p->init, p->seed, N, R, ld, X, Y, ctx->X1, s));
+    if (has_prologue(p->variant)) HIP_TRY(launch_asb_local(prologue_coeffs(*p), N, R, ld, X, Y, s));
+    const size_t tb = sizeof(float) * (size_t)N * ldr;
+    HIP_TRY(hipMemsetAsync(ctx->XT0, 0, tb, s));
+    HIP_TRY(hipMemsetAsync(ctx->XT1, 0, tb, s));
+    HIP_TRY(hipMemsetAsync(ctx->YT, 0, tb, s));
+    HIP_TRY(launch_to_spin_major(X, ld, N, R, ctx->XT0, ldr, s));
+    HIP_TRY(launch_to_spin_major(Y, ld, N, R, ctx->YT, ldr, s));
+    SparseArgs a = sparse_args(prob, R, ldr);
+    float *XT = ctx->XT0, *XTn = ctx->XT1;
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    for (int k = 0; k < p->steps; ++k) {
+        sb_step_coeffs_for(p, c0, k, &a.c);
+        a.XT = XT;
+        a.XTn = XTn;
+        a.YT = ctx->YT;
+        hipError_t e = launch_step_sparse(a, s);
+        if (e != hipSuccess) return hip_fail(e, "launch_step_sparse");
+        std::swap(XT, XTn);
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+    HIP_TRY(launch_to_replica_major(XT, ldr, N, R, X, ld, s));
+    HIP_TRY(launch_to_replica_major(ctx->YT, ldr, N, R, Y, ld, s));
+    rc = read_results(ctx, prob, X, Y, R, res, s);
+    if (rc) return rc;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res->step_ms = ms;
+    res->c0_used = c0;
+    return SB_OK;
+}
+
 int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)
 {
+    if (prob->nnz > 0 && p->precision != SB_PREC_ORDERED) return solve_sparse(ctx, prob, p, c0, res);
     const int N = prob->N, ld = prob->ld, R = p->replicas;
     hipStream_t s = ctx->stream;
     int family = -1, shape = -1;
@@ -1043,6 +1185,21 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
     if (precision == SB_PREC_ORDERED) {
         a.J = prob->J32;
         e = launch_step_ordered(a, s);
+    } else if (prob->nnz > 0) {
+        // Sparse family: one step on the spin-major copy of X (Y = 0).
+        const int ldr = round_up(R, 64);
+        if ((rc = sbh::ensure_sparse_ws(ctx, (size_t)N * ldr))) return rc;
+        const size_t tb = sizeof(float) * (size_t)N * ldr;
+        HIP_TRY(hipMemsetAsync(ctx->XT0, 0, tb, s));
+        HIP_TRY(hipMemsetAsync(ctx->YT, 0, tb, s));
+        HIP_TRY(launch_to_spin_major(ctx->X0, ld, N, R, ctx->XT0, ldr, s));
+        SparseArgs sa = sbh::sparse_args(prob, R, ldr);
+        sa.c = a.c;
+        sa.XT = ctx->XT0;
+        sa.XTn = ctx->XT1;
+        sa.YT = ctx->YT;
+        e = launch_step_sparse(sa, s);
+        if (e == hipSuccess) e = launch_to_replica_major(ctx->YT, ldr, N, R, ctx->Y, ld, s);
     } else {
         int family = 0, shape = 0;
         rc = sbh::choose_kernel(ctx, prob, precision, a.c.variant, R, &family, &shape);

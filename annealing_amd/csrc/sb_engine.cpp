@@ -62,6 +62,10 @@ size_t Engine::state_bytes(int N, int R)
 int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
 {
     const int N = prob->N, ld = prob->ld, R = p.replicas;
+    if (prob->nnz > 0) {
+        set_error("batched solves take dense problems (sb_problem_set_sparse(prob, 0))");
+        return SB_E_INVALID;
+    }
     const int family = engine_family(p.precision, p.variant, prob, R);
     if (family < 0) {
         set_error("batched solve supports SB_PREC_FP32 and SB_PREC_SPLIT");

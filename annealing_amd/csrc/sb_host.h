@@ -26,6 +26,9 @@ struct sb_ctx {
     size_t S_cap = 0;
     float *tmp = nullptr;
     size_t tmp_cap = 0;
+    // Spin-major state of the sparse family ([N][round_up(R, 64)] each).
+    float *XT0 = nullptr, *XT1 = nullptr, *YT = nullptr;
+    size_t t_cap = 0;
     // Autotuned (family, shape) per (family set, N bucket, R bucket).
     std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tune;
     bool autotune = true;
@@ -43,6 +46,10 @@ struct sb_problem {
     uint8_t *Jfp8 = nullptr;  // only when integer and |J| <= 16
     float *h = nullptr;       // external field [ld] (zero padded), or null
     bool h_integer = true;
+    // CSR form of J (sb_problem_set_sparse): solves then use the sparse family.
+    int *csr_ptr = nullptr, *csr_col = nullptr;
+    float *csr_w = nullptr;
+    long long nnz = 0;
 };
 
 namespace sbh {
@@ -58,6 +65,9 @@ int check_params(const sb_params *p, int N, double sigma, double *c0_out);
 // first fused step (aSB: M sub-steps, m2_opt.cu:168 with s = 0; aSB-m1: the
 // x-step, Bifurcation_m1.cu.c:166).  has_prologue is false for bSB/dSB.
 bool has_prologue(int variant);
+// Sparse family helpers (spin-major workspace, kernel arguments of a problem).
+int ensure_sparse_ws(sb_ctx *ctx, size_t elems);
+sbk::SparseArgs sparse_args(const sb_problem *prob, int R, int ldr);
 sb_step_coeffs prologue_coeffs(const sb_params &p);
 
 // Pick the kernel family for a precision/variant/problem (before tuning).

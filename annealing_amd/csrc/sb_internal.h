@@ -101,6 +101,24 @@ TileShape shape_at(int family, int idx);
 // Launchers (sb_kernels.hip).  Return a hipError_t.
 hipError_t launch_step(int family, int shape_idx, const StepArgs &a, hipStream_t s);
 hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s);
+
+// Sparse family: CSR J (rows in ascending column order) and a spin-major state
+// XT, YT [N][ldr] (ldr = round_up(R, 64); replicas of one spin contiguous).
+struct SparseArgs {
+    const int *ptr;    // [N + 1] row starts
+    const int *col;    // [nnz] column indices
+    const float *w;    // [nnz] values
+    const float *h;    // external field [N] or null
+    const float *XT;   // state read this step
+    float *XTn;        // state written this step
+    float *YT;         // updated in place
+    int N, R, ldr;
+    sb_step_coeffs c;
+};
+hipError_t launch_step_sparse(const SparseArgs &a, hipStream_t s);
+// [R][ldx] replica-major <-> [N][ldr] spin-major (rows/columns past R or N untouched).
+hipError_t launch_to_spin_major(const float *X, int ldx, int N, int R, float *XT, int ldr, hipStream_t s);
+hipError_t launch_to_replica_major(const float *XT, int ldr, int N, int R, float *X, int ldx, hipStream_t s);
 // Planes of kind `kind` for state X ([R][ldx]) into B.
 hipError_t launch_make_planes(int kind, const float *X, int R, int ldx, void *B, hipStream_t s);
 hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X, float *Y,

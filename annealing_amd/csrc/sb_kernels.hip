@@ -1041,6 +1041,82 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Sparse family (CSR J, spin-major state).  For low-density J the field is a
+// gather: wave w of workgroup t owns spin i = 4t + w and the 64 replicas
+// 64 blockIdx.y + lane (lane = replica).  For each nonzero (j, w_ij) of row i,
+// in ascending j, the wave reads the 64 consecutive replicas of X^T[j] (one
+// 256-byte load) and accumulates fmaf(w_ij, x_j, acc): the sequential
+// j = 0..N-1 chain of Bifurcation_m2.cu.c:93-97 without its zero terms (a zero
+// term changes at most the sign of a zero sum), so it equals the ordered path.
+// The row's indices and values are wave-uniform (scalar loads).
+
+__global__ __launch_bounds__(256) void k_step_sparse(SparseArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= a.N) return;
+    const int r = blockIdx.y * 64 + lane;  // < ldr: padding replicas hold 0
+    const bool sgn = a.c.variant == SB_DSB;
+    const int k0 = a.ptr[i], k1 = a.ptr[i + 1];
+    const float *xr = a.XT + r;
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int k = k0; k < k1; ++k) {
+        const int j = a.col[k];
+        float v = xr[(size_t)j * a.ldr];
+        if (sgn) v = sgn1(v);
+        acc = __builtin_fmaf(a.w[k], v, acc);
+    }
+    if (r >= a.R) return;
+    if (a.h) acc = acc + a.h[i];  // external field, after the sum
+    const size_t o = (size_t)i * a.ldr + r;
+    float x = a.XT[o], y = a.YT[o];
+    sb_update(a.c, acc, x, y);
+    a.XTn[o] = x;
+    a.YT[o] = y;
+}
+
+hipError_t launch_step_sparse(const SparseArgs &a, hipStream_t s)
+{
+    if (a.N <= 0 || a.R <= 0) return hipSuccess;
+    if (a.ldr % 64 || a.ldr < a.R) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_step_sparse, dim3((a.N + 3) / 4, a.ldr / 64), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// 32 x 32 tiles through LDS (one side of each copy coalesced on each pass).
+__global__ __launch_bounds__(256) void k_transpose(const float *src, int lds_, int rows, int cols, float *dst,
+                                                   int ldd)
+{
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) t[k][tx] = src[(size_t)r * lds_ + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;  // dst row c, column r
+        if (r < rows && c < cols) dst[(size_t)c * ldd + r] = t[tx][k];
+    }
+}
+
+hipError_t launch_to_spin_major(const float *X, int ldx, int N, int R, float *XT, int ldr, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_transpose, dim3((N + 31) / 32, (R + 31) / 32), dim3(256), 0, s, X, ldx, R, N, XT, ldr);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_replica_major(const float *XT, int ldr, int N, int R, float *X, int ldx, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_transpose, dim3((R + 31) / 32, (N + 31) / 32), dim3(256), 0, s, XT, ldr, N, R, X, ldx);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
 {
     if (a.N <= 0 || a.R <= 0) return hipSuccess;

@@ -35,7 +35,8 @@ EXPORTS = (
     "sb_edgelist_parse", "sb_edgelist_parse_field", "sb_instance_stats", "sb_c0_auto", "sb_generate",
     "sb_ctx_create", "sb_ctx_destroy", "sb_problem_create",
     "sb_problem_from_edgelist_file", "sb_problem_generate", "sb_problem_read_j",
-    "sb_problem_destroy", "sb_problem_info", "sb_problem_set_field",
+    "sb_problem_destroy", "sb_problem_info", "sb_problem_set_field", "sb_problem_set_sparse",
+    "sb_problem_sparse_nnz",
     "sb_solve", "sb_step_coeffs_for", "sb_step", "sb_field", "sb_init_state", "sb_asb_prologue",
     "sb_energy", "sb_batch_solve", "sb_server_create", "sb_server_submit",
     "sb_server_submit_generated", "sb_server_wait", "sb_server_cancel", "sb_server_stats_get",
@@ -119,6 +120,8 @@ def _bind(lib: C.CDLL) -> None:
         "sb_edgelist_parse_field": (C.c_int, [C.c_char_p, SZ, C.c_int, C.POINTER(I32), P, I32, P,
                                               C.POINTER(I32)]),
         "sb_problem_set_field": (C.c_int, [P, P]),
+        "sb_problem_set_sparse": (C.c_int, [P, I32]),
+        "sb_problem_sparse_nnz": (C.c_int, [P, C.POINTER(C.c_int64)]),
         "sb_instance_stats": (C.c_int, [I32, P, I32, C.POINTER(D), C.POINTER(I32), C.POINTER(D)]),
         "sb_c0_auto": (D, [I32, I32, D, D]),
         "sb_generate": (C.c_int, [I32, I32, D, U64, P]),
@@ -307,6 +310,16 @@ class Problem:
         h = np.ascontiguousarray(h, dtype=np.float32)
         assert h.shape == (self.N,)
         _check(load().sb_problem_set_field(self.handle, _ptr(h)), "sb_problem_set_field")
+
+    def set_sparse(self, enable: bool = True) -> None:
+        """Build (or free) the CSR form; solves then run the sparse family."""
+        _check(load().sb_problem_set_sparse(self.handle, int(bool(enable))), "sb_problem_set_sparse")
+
+    @property
+    def nnz(self) -> int:
+        n = C.c_int64()
+        _check(load().sb_problem_sparse_nnz(self.handle, C.byref(n)), "sb_problem_sparse_nnz")
+        return n.value
 
     def read_j(self) -> np.ndarray:
         J = np.empty((self.N, self.N), dtype=np.float32)

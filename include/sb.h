@@ -181,6 +181,15 @@ SB_API int sb_problem_from_edgelist_file(sb_ctx *ctx, const char *path, int one_
  * int64 when J and h are integers).  sb_step and sb_energy on caller buffers
  * take no field. */
 SB_API int sb_problem_set_field(sb_problem *prob, const float *h);
+/* Sparse form (enable != 0): builds a CSR copy of J (rows in ascending column
+ * order); sb_solve and sb_field with SB_PREC_FP32 or SB_PREC_SPLIT then use
+ * the sparse family (a gather per nonzero on a spin-major state, FP32 fmaf
+ * over the nonzeros in ascending column order: the same sums as
+ * SB_PREC_ORDERED).  Pays off below a few percent density.  Batched solves and
+ * the server take dense problems only (SB_E_INVALID).  enable = 0 frees it. */
+SB_API int sb_problem_set_sparse(sb_problem *prob, int32_t enable);
+/* Nonzeros of the sparse form (0 when the problem is dense only). */
+SB_API int sb_problem_sparse_nnz(const sb_problem *prob, int64_t *nnz);
 /* Generate a synthetic instance on the device (kinds as sb_generate; the
  * same J as sb_generate gives on the host, up to rare last-bit differences of
  * libm in the SK kinds). */

@@ -138,6 +138,40 @@ def leg_matvec(sb, ctx, S):
     return leg
 
 
+def leg_sparse(sb, ctx, S):
+    """Low-density instances: the CSR (sparse) family against the dense paths on
+    the same J and seeds.  The sparse sums are the sequential order; the dense
+    paths are MFMA tiles (FP32, and the split form on FP8 J)."""
+    leg = {"workload": f"ER +-1 (J = -W) generated on the device, bSB, {S} steps, dt=0.5, seed 5"}
+    for N, p, R in ((16384, 0.01, 64), (4096, 0.05, 64), (4096, 0.01, 256)):
+        prob = ctx.generate(sb.GEN_ER, N, p, seed=5)
+        kw = dict(spins=False, variant="bsb", steps=S, replicas=R, init="uniform", seed=1, dt=0.5)
+        prob.set_sparse(True)
+        nnz = prob.nnz
+        ctx.solve(prob, precision="fp32", **kw)
+        sp = ctx.solve(prob, precision="fp32", **kw)
+        prob.set_sparse(False)
+        out = {"nnz": nnz, "sparse_fp32_us": sp.step_ms * 1e3 / S,
+               "sparse_mean_energy": float(sp.energy.mean())}
+        for prec in ("fp32", "split"):
+            ctx.solve(prob, precision=prec, **kw)
+            d = ctx.solve(prob, precision=prec, **kw)
+            out[f"dense_{prec}_us"] = d.step_ms * 1e3 / S
+            out[f"dense_{prec}_mean_energy"] = float(d.energy.mean())
+        us = out["sparse_fp32_us"]
+        # CSR (4 B index + 4 B value) and the state (16 B per spin-replica) from HBM;
+        # one 256-byte gather of X^T per nonzero and 64-replica block from L2
+        out["hbm_gbs_alg"] = (8.0 * nnz + 16.0 * N * R) / (us * 1e-6) / 1e9
+        out["gather_gbs"] = nnz * 4.0 * ((R + 63) // 64 * 64) / (us * 1e-6) / 1e9
+        out["speedup_vs_dense_split"] = out["dense_split_us"] / us
+        out["speedup_vs_dense_fp32"] = out["dense_fp32_us"] / us
+        leg[f"er{N}_p{p}_r{R}"] = out
+        log(f"sparse N={N} p={p} R={R}: {us:.1f} us/step vs dense split {out['dense_split_us']:.1f}, "
+            f"fp32 {out['dense_fp32_us']:.1f}")
+        prob.close()
+    return leg
+
+
 def leg_batched(sb, ctx, n_inst, steps):
     rng = np.random.default_rng(4)
     Ns = rng.integers(64, 4097, n_inst)
@@ -308,6 +342,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy leg")
     ap.add_argument("--no-matvec", action="store_true", help="skip the R=1 matrix-vector leg")
+    ap.add_argument("--no-sparse", action="store_true", help="skip the low-density (CSR) leg")
     ap.add_argument("--only-headline", action="store_true", help="skip every extra leg")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=1500)
@@ -334,6 +369,7 @@ def main() -> int:
     if args.only_headline:
         args.no_cpu = args.no_precision = args.no_batched = args.no_serving = args.no_accuracy = True
         args.no_matvec = True
+        args.no_sparse = True
 
     from annealing_amd import sb
 
@@ -378,6 +414,8 @@ def main() -> int:
         out["precision_form"] = leg_precision(sb, ctx, S)
     if not args.no_matvec:
         out["matvec"] = leg_matvec(sb, ctx, 200)
+    if not args.no_sparse:
+        out["sparse"] = leg_sparse(sb, ctx, 200)
     if not args.no_batched:
         out["batched"] = leg_batched(sb, ctx, args.batch_instances, args.batch_steps)
     if not args.no_serving:

@@ -31,6 +31,7 @@ static void usage()
         "  --init reference|uniform  initial state (default reference)\n"
         "  --seed S                  (default 0)\n"
         "  --one-based               edge list indices start at 1\n"
+        "  --sparse                  CSR field (low-density instances; same sums as ordered)\n"
         "  --output FILE             energies (default output.txt)\n"
         "  --spins FILE              also write spins, one replica per line\n"
         "  --device D                GPU ordinal (default 0)\n");
@@ -66,7 +67,7 @@ int main(int argc, char **argv)
     sb_default_params(&p);
     const char *instance = nullptr;
     std::string out_path = "output.txt", spins_path;
-    int one_based = 0, device = 0;
+    int one_based = 0, device = 0, sparse = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto need = [&](const char *name) -> const char * {
@@ -110,6 +111,7 @@ int main(int argc, char **argv)
         else if (a == "--c0" && parse_double(need("--c0"), &d)) p.c0 = d;
         else if (a == "--K" && parse_double(need("--K"), &d)) p.K = d;
         else if (a == "--one-based") one_based = 1;
+        else if (a == "--sparse") sparse = 1;
         else if (a == "--output") out_path = need("--output");
         else if (a == "--spins") spins_path = need("--spins");
         else if (a == "--help" || a == "-h") { usage(); return 0; }
@@ -131,6 +133,12 @@ int main(int argc, char **argv)
     rc = sb_problem_from_edgelist_file(ctx, instance, one_based, &prob);
     if (rc) {
         std::fprintf(stderr, "%s: %s (%s)\n", instance, sb_status_string(rc), sb_last_error_message());
+        sb_ctx_destroy(ctx);
+        return 1;
+    }
+    if (sparse && (rc = sb_problem_set_sparse(prob, 1)) != SB_OK) {
+        std::fprintf(stderr, "--sparse: %s (%s)\n", sb_status_string(rc), sb_last_error_message());
+        sb_problem_destroy(prob);
         sb_ctx_destroy(ctx);
         return 1;
     }

@@ -237,14 +237,15 @@ def test_invalid_params_rejected(ctx):
         assert e.value.status == 1
 
 
-def test_cli_writes_output_txt(ctx, orc):
+@pytest.mark.parametrize("mode", [["--precision", "ordered"], ["--precision", "fp32", "--sparse"]])
+def test_cli_writes_output_txt(ctx, orc, mode):
     J = kn_pm1(40, 77)
     lines = ["40"] + [f"{i} {j} {int(J[i, j])}" for i in range(40) for j in range(i + 1, 40)]
     exe = os.path.join(ROOT, "annealing_amd", "bin", "sb_solve")
     with tempfile.TemporaryDirectory() as d:
         inst = os.path.join(d, "k40.txt")
         open(inst, "w").write("\n".join(lines) + "\n")
-        r = subprocess.run([exe, inst, "--variant", "bsb", "--precision", "ordered", "--steps", "200",
+        r = subprocess.run([exe, inst, "--variant", "bsb", *mode, "--steps", "200",
                             "--replicas", "6", "--init", "uniform", "--seed", "3", "--dt", "0.5",
                             "--spins", "spins.txt"], cwd=d, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr

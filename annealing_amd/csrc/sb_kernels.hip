@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace sbk {
 
@@ -102,12 +103,15 @@ __device__ __forceinline__ float m1_y(float F, float x, float y, float p, double
     return ny;
 }
 
-__device__ __forceinline__ void sb_update(const sb_step_coeffs &c, float F, float &x, float &y)
+// The update of one spin-replica for a compile-time variant (bSB and dSB share
+// it: they differ only in the field).
+template <int V>
+__device__ __forceinline__ void sb_update_v(const sb_step_coeffs &c, float F, float &x, float &y)
 {
-    if (c.variant == SB_ASB_M1) {
+    if constexpr (V == SB_ASB_M1) {
         y = m1_y(F, x, y, c.p_next, c.a0, c.K, c.dts, c.c0);
         if (c.do_local) x = m1_x(x, y, c.a0, c.dts);
-    } else if (c.variant == SB_ASB) {
+    } else if constexpr (V == SB_ASB) {
         y = __builtin_fmaf(c.beta, F, y);  // cublasSgemv(alpha = dt xi, beta = 1)
         if (c.do_local) asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
     } else {
@@ -120,6 +124,16 @@ __device__ __forceinline__ void sb_update(const sb_step_coeffs &c, float F, floa
         x = xn;
         y = yn;
     }
+}
+
+__device__ __forceinline__ void sb_update(const sb_step_coeffs &c, float F, float &x, float &y)
+{
+    if (c.variant == SB_ASB_M1)
+        sb_update_v<SB_ASB_M1>(c, F, x, y);
+    else if (c.variant == SB_ASB)
+        sb_update_v<SB_ASB>(c, F, x, y);
+    else
+        sb_update_v<SB_BSB>(c, F, x, y);
 }
 
 __device__ __forceinline__ float sgn1(float v) { return v > 0.0f ? 1.0f : -1.0f; }
@@ -184,8 +198,9 @@ __device__ __forceinline__ void write_plane1(int kind, float x, void *B, size_t 
 // rep_base + 8*(v/4) + 4*(l/32) + v%4 and spin spin_base + l%32.  Every load
 // and store of a half-wave then covers 32 consecutive spins (128 bytes of X or
 // Y) of one replica: whole cache lines, not 16-byte pieces of 32 lines.
-__device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &acc, int spin_base, int rep_base,
-                                               int lane)
+template <int V>
+__device__ __forceinline__ void epilogue_block_v(const StepArgs &a, const v16f &acc, int spin_base, int rep_base,
+                                                 int lane)
 {
     const int i = spin_base + (lane & 31);
     if (i >= a.N) return;
@@ -205,7 +220,7 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
         }
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            sb_update(a.c, field(acc[v]), xv[v], yv[v]);
+            sb_update_v<V>(a.c, field(acc[v]), xv[v], yv[v]);
             a.Xn[off(v)] = xv[v];
             a.Y[off(v)] = yv[v];
         }
@@ -219,11 +234,41 @@ __device__ __forceinline__ void epilogue_block(const StepArgs &a, const v16f &ac
     for (int v = 0; v < 16; ++v) {
         if (r0 + 8 * (v >> 2) + (v & 3) >= a.R) continue;
         float x = a.X[off(v)], y = a.Y[off(v)];
-        sb_update(a.c, field(acc[v]), x, y);
+        sb_update_v<V>(a.c, field(acc[v]), x, y);
         a.Xn[off(v)] = x;
         a.Y[off(v)] = y;
         if (a.Bn) write_plane1(a.bkind, x, a.Bn, a.bplane, off(v));
     }
+}
+
+// Epilogue of a wave's BM x BN accumulator blocks.  The variant is dispatched
+// once, outside the blocks, and the blocks are expanded at compile time (a
+// fold over an index sequence, not a loop left to the unroller), so every
+// accumulator is indexed statically and stays in registers at any BM x BN.
+template <int BM, int BN, int V, int... I>
+__device__ __forceinline__ void epilogue_blocks_seq(const StepArgs &a, const v16f (&acc)[BM][BN], int spin_base,
+                                                    int rep_base, int lane, std::integer_sequence<int, I...>)
+{
+    (epilogue_block_v<V>(a, acc[I / BN][I % BN], spin_base + 32 * (I / BN), rep_base + 32 * (I % BN), lane), ...);
+}
+
+template <int BM, int BN, int V>
+__device__ __forceinline__ void epilogue_blocks_v(const StepArgs &a, const v16f (&acc)[BM][BN], int spin_base,
+                                                  int rep_base, int lane)
+{
+    epilogue_blocks_seq<BM, BN, V>(a, acc, spin_base, rep_base, lane, std::make_integer_sequence<int, BM * BN>{});
+}
+
+template <int BM, int BN>
+__device__ __forceinline__ void epilogue_tile(const StepArgs &a, const v16f (&acc)[BM][BN], int spin_base,
+                                              int rep_base, int lane)
+{
+    if (a.c.variant == SB_ASB_M1)
+        epilogue_blocks_v<BM, BN, SB_ASB_M1>(a, acc, spin_base, rep_base, lane);
+    else if (a.c.variant == SB_ASB)
+        epilogue_blocks_v<BM, BN, SB_ASB>(a, acc, spin_base, rep_base, lane);
+    else
+        epilogue_blocks_v<BM, BN, SB_BSB>(a, acc, spin_base, rep_base, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -395,11 +440,7 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
         }
     };
     kpipeline((Kp + KC - 1) / KC, gload, sstore, compute);
-#pragma unroll
-    for (int bm = 0; bm < BM; ++bm)
-#pragma unroll
-        for (int bn = 0; bn < BN; ++bn)
-            epilogue_block(a, acc[bm][bn], row0 + wm * 32 * BM + 32 * bm, col0 + wn * 32 * BN + 32 * bn, lane);
+    epilogue_tile<BM, BN>(a, acc, row0 + wm * 32 * BM, col0 + wn * 32 * BN, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,11 +628,7 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
         }
     };
     kpipeline((Kp + KC - 1) / KC, gload, sstore, compute);
-#pragma unroll
-    for (int bm = 0; bm < BM; ++bm)
-#pragma unroll
-        for (int bn = 0; bn < BN; ++bn)
-            epilogue_block(a, acc[bm][bn], row0 + wm * 32 * BM + 32 * bm, col0 + wn * 32 * BN + 32 * bn, lane);
+    epilogue_tile<BM, BN>(a, acc, row0 + wm * 32 * BM, col0 + wn * 32 * BN, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -732,11 +769,7 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
         }
     };
     kpipeline((Kp + KC - 1) / KC, gload, sstore, compute);
-#pragma unroll
-    for (int bm = 0; bm < BM; ++bm)
-#pragma unroll
-        for (int bn = 0; bn < BN; ++bn)
-            epilogue_block(a, acc[bm][bn], row0 + wm * 32 * BM + 32 * bm, col0 + wn * 32 * BN + 32 * bn, lane);
+    epilogue_tile<BM, BN>(a, acc, row0 + wm * 32 * BM, col0 + wn * 32 * BN, lane);
 }
 
 // ---------------------------------------------------------------------------

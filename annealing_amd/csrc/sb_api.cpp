@@ -367,12 +367,13 @@ int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N)
         if ((rc = grow(&ctx->Y, &c2, state_elems))) return rc;
         ctx->state_cap = state_elems;
     }
-    if (ctx->plane_cap < 4 * state_elems) {
+    const size_t plane_elems = plane_bytes(BK_SIGN_BF16) * state_elems;  // the widest plane kind
+    if (ctx->plane_cap < plane_elems) {
         (void)hipStreamSynchronize(ctx->stream);
         size_t c0 = ctx->plane_cap, c1 = ctx->plane_cap;
-        if ((rc = grow(&ctx->P0, &c0, 4 * state_elems))) return rc;
-        if ((rc = grow(&ctx->P1, &c1, 4 * state_elems))) return rc;
-        ctx->plane_cap = 4 * state_elems;
+        if ((rc = grow(&ctx->P0, &c0, plane_elems))) return rc;
+        if ((rc = grow(&ctx->P1, &c1, plane_elems))) return rc;
+        ctx->plane_cap = plane_elems;
     }
     if ((rc = grow(&ctx->H, &ctx->H_cap, (size_t)std::max(R, 1)))) return rc;
     if ((rc = grow(&ctx->part, &ctx->part_cap, std::max<size_t>(energy_partial_elems(N, R), 1)))) return rc;
@@ -799,7 +800,6 @@ static int tune_candidates(sb_ctx *ctx, const sb_problem *prob, int variant, int
     a.R = R;
     a.c.variant = variant;
     a.c.M = 1;
-    a.bplane = (size_t)R * prob->ld;
     float best_ms = 1e30f;
     *best = cands.front();
     const int reps = 3;
@@ -1050,7 +1050,6 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     a.N = N;
     a.R = R;
     a.bkind = p->precision == SB_PREC_ORDERED ? BK_NONE : plane_kind(family, p->variant);
-    a.bplane = (size_t)R * ld;
     unsigned char *P = ctx->P0, *Pn = ctx->P1;
     // Both plane buffers start complete: the epilogue rewrites only the
     // 4-spin groups holding real spins, and the K loop reads the padding

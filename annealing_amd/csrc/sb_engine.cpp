@@ -213,7 +213,6 @@ int Engine::rebuild()
         I.Y = s.Y;
         I.P[0] = s.P[0];
         I.P[1] = s.P[1];
-        I.bplane = (size_t)R * s.prob->ld;
         I.bkind = s.bkind;
         I.ldx = s.prob->ld;
         I.N = N;

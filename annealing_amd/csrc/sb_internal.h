@@ -19,12 +19,15 @@ constexpr int kSpinPad = 64;
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // Arguments of one fused step over one instance.
-// B-operand planes: the field kernels' B operand prepared once per step by
-// the previous step's epilogue (or k_make_planes after init), so staging it
-// is a plain copy.  Layout [R][ldx] per plane.
+// B-operand planes: the sign of X in the MFMA operand format, written once per
+// step by the previous step's epilogue (or k_make_planes after init), so
+// staging it is a plain copy of 1-2 bytes per element instead of 4.  Layout
+// [R][ldx].  The split bSB/aSB operand (BF16 hi and lo of x) is not kept as
+// planes: the K loop reads FP32 X (the same 4 bytes per element as hi + lo)
+// and splits it while staging, so the epilogue writes 16 bytes per element
+// instead of 20.
 enum BKind {
-    BK_NONE = 0,      // B staged from X (FP32 family, or split without planes)
-    BK_HILO = 1,      // BF16 hi plane, then BF16 lo plane (bplane elements later)
+    BK_NONE = 0,      // B staged from X
     BK_SIGN_FP8 = 2,  // E4M3 +-1 bytes
     BK_SIGN_BF16 = 3  // BF16 +-1
 };
@@ -37,7 +40,6 @@ struct StepArgs {
     const float *h;  // external field [N] added to J.x (null: none)
     const void *B;   // B-operand planes of X (bkind != BK_NONE)
     void *Bn;        // planes of Xn written by the epilogue (may be null)
-    size_t bplane;   // elements per plane (R * ldx)
     int bkind;
     int ldj, ldx;
     int N, R;
@@ -55,14 +57,14 @@ int rows_per_group(int tiles_m, int tiles_n, int mt, int nt, int bytes_j, int by
 // Plane kind a family reads for a variant (BK_NONE: stages from X).
 inline int plane_kind(int family, int variant)
 {
-    if (family == 1 || family == 2) return variant == SB_DSB ? BK_SIGN_BF16 : BK_HILO;
+    if (family == 1 || family == 2) return variant == SB_DSB ? BK_SIGN_BF16 : BK_NONE;
     if (family == 3) return BK_SIGN_FP8;
     return BK_NONE;
 }
 // Bytes per state element of a plane set of this kind.
 inline size_t plane_bytes(int kind)
 {
-    return kind == BK_HILO ? 4 : kind == BK_SIGN_BF16 ? 2 : kind == BK_SIGN_FP8 ? 1 : 0;
+    return kind == BK_SIGN_BF16 ? 2 : kind == BK_SIGN_FP8 ? 1 : 0;
 }
 
 // Tile shapes.  A workgroup has WM x WN waves; each wave owns BM x BN blocks
@@ -145,8 +147,7 @@ struct BatchInstance {
     const float *h;     // external field [N] or null
     float *X[2];        // ping-pong state, read X[k & 1], write X[(k + 1) & 1]
     float *Y;
-    void *P[2];         // B-operand planes (split families), same ping-pong
-    size_t bplane;
+    void *P[2];         // B-operand sign planes (dSB), same ping-pong
     int bkind;
     int ldx;
     int N, R;

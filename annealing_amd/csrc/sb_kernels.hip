@@ -148,23 +148,11 @@ __device__ __forceinline__ float bf2f(unsigned short h)
 }
 
 // B-operand planes of four consecutive spins (one float4 of X) at element o:
-//   BK_HILO      hi = bf16(x), lo = bf16(x - hi): x = hi + lo to 2^-17 |x|
 //   BK_SIGN_FP8  E4M3 +1.0 = 0x38, -1.0 = 0xB8 (x > 0 ? +1 : -1)
 //   BK_SIGN_BF16 BF16 +1.0 = 0x3F80, -1.0 = 0xBF80
-__device__ __forceinline__ void write_planes(int kind, const v4f &xv, void *B, size_t bplane, size_t o)
+__device__ __forceinline__ void write_planes(int kind, const v4f &xv, void *B, size_t o)
 {
-    if (kind == BK_HILO) {
-        u16x4 h, l;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            unsigned short hb = f2bf(xv[e]);
-            h[e] = hb;
-            l[e] = f2bf(xv[e] - bf2f(hb));
-        }
-        unsigned short *p = (unsigned short *)B;
-        *(u16x4 *)(p + o) = h;
-        *(u16x4 *)(p + bplane + o) = l;
-    } else if (kind == BK_SIGN_FP8) {
+    if (kind == BK_SIGN_FP8) {
         unsigned int s4 = 0u;
 #pragma unroll
         for (int e = 0; e < 4; ++e) s4 |= (xv[e] > 0.0f ? 0x38u : 0xB8u) << (8 * e);
@@ -178,14 +166,9 @@ __device__ __forceinline__ void write_planes(int kind, const v4f &xv, void *B, s
 }
 
 // Plane entry of one spin (same encodings as write_planes).
-__device__ __forceinline__ void write_plane1(int kind, float x, void *B, size_t bplane, size_t o)
+__device__ __forceinline__ void write_plane1(int kind, float x, void *B, size_t o)
 {
-    if (kind == BK_HILO) {
-        unsigned short *p = (unsigned short *)B;
-        const unsigned short hb = f2bf(x);
-        p[o] = hb;
-        p[bplane + o] = f2bf(x - bf2f(hb));
-    } else if (kind == BK_SIGN_FP8) {
+    if (kind == BK_SIGN_FP8) {
         ((unsigned char *)B)[o] = x > 0.0f ? (unsigned char)0x38 : (unsigned char)0xB8;
     } else if (kind == BK_SIGN_BF16) {
         ((unsigned short *)B)[o] = x > 0.0f ? (unsigned short)0x3f80 : (unsigned short)0xbf80;
@@ -226,7 +209,7 @@ __device__ __forceinline__ void epilogue_block_v(const StepArgs &a, const v16f &
         }
         if (a.Bn) {
 #pragma unroll
-            for (int v = 0; v < 16; ++v) write_plane1(a.bkind, xv[v], a.Bn, a.bplane, off(v));
+            for (int v = 0; v < 16; ++v) write_plane1(a.bkind, xv[v], a.Bn, off(v));
         }
         return;
     }
@@ -237,7 +220,7 @@ __device__ __forceinline__ void epilogue_block_v(const StepArgs &a, const v16f &
         sb_update_v<V>(a.c, field(acc[v]), x, y);
         a.Xn[off(v)] = x;
         a.Y[off(v)] = y;
-        if (a.Bn) write_plane1(a.bkind, x, a.Bn, a.bplane, off(v));
+        if (a.Bn) write_plane1(a.bkind, x, a.Bn, off(v));
     }
 }
 
@@ -448,10 +431,10 @@ __device__ __forceinline__ void fp32_tile(const StepArgs &a, int tile, float *sm
 // as BF16, or as FP8 E4M3 widened to BF16 by v_cvt_scalef32_pk_bf16_fp8 while
 // staging).  B = X as BF16 hi = bf16(x) and lo = bf16(x - hi), two MFMA passes
 // into one FP32 accumulator per k16 group (hi then lo); dSB uses sign(x) once.
-// With PLANES the B operand comes ready-made from planes written by the
-// previous step's epilogue; without, it is split from X while staging (same
-// values).  Lane half h reads 8 consecutive k (16 bytes) at 16kk + 8h for A
-// and B.
+// B is split from FP32 X while staging; for dSB (SGN), PLANES instead copies
+// the BF16 sign plane written by the previous step's epilogue (2 bytes per
+// element instead of 4; same values).  Lane half h reads 8 consecutive k
+// (16 bytes) at 16kk + 8h for A and B.
 
 // 4 FP8 bytes -> 4 BF16 (exact for the integers |J| <= 16 the packer stores).
 __device__ __forceinline__ void fp8x4_to_bf16x4(unsigned int b4, unsigned int &lo2, unsigned int &hi2)
@@ -469,6 +452,7 @@ struct SplitTile {
 template <int WM, int WN, int BM, int BN, bool FP8J, bool PLANES, bool SGN>
 __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned short *smem)
 {
+    static_assert(!PLANES || SGN, "planes hold signs only");
     using T = SplitTile<WM, WN, BM, BN>;
     constexpr int MT = T::MT, NT = T::NT, NTHR = T::NTHR, KC = T::KC, LD = T::LDS_LD;
     // A: 16-byte global loads of 8 BF16 or 16 FP8.
@@ -490,23 +474,19 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
     const int wm = wave % WM, wn = wave / WM;
     const int Kp = round_up_dev(a.N, KC);
     const unsigned short *Ph = (const unsigned short *)a.B;
-    const unsigned short *Pl = Ph + a.bplane;
     // Rows row0.. of J and replicas col0.. of the B source; loads past N or R
     // read zeros.
     constexpr int JBYTES = FP8J ? 1 : 2;
     const int rows = min(MT, a.N - row0), cols = min(NT, a.R - col0);
     const __amdgpu_buffer_rsrc_t rJ = tile_rsrc((const unsigned char *)a.J + (size_t)row0 * a.ldj * JBYTES,
                                                 (long long)rows * a.ldj * JBYTES);
-    const __amdgpu_buffer_rsrc_t rBh =
+    const __amdgpu_buffer_rsrc_t rB =
         PLANES ? tile_rsrc(Ph + (size_t)col0 * a.ldx, (long long)cols * a.ldx * 2)
                : tile_rsrc(a.X + (size_t)col0 * a.ldx, (long long)cols * a.ldx * 4);
-    const __amdgpu_buffer_rsrc_t rBl =
-        tile_rsrc(PLANES && !SGN ? Pl + (size_t)col0 * a.ldx : Ph, PLANES && !SGN ? (long long)cols * a.ldx * 2 : 0);
 
     struct Stage {
         u32x4 ra[LA];
-        u32x4 rb[PLANES ? 1 : LB];
-        u32x4 rbh[PLANES ? LB : 1], rbl[PLANES ? LB : 1];
+        u32x4 rb[LB];
     };
     Stage st0, st1;
     auto gload = [&](auto S, int ch) {
@@ -520,13 +500,10 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             const int q = tid + u * NTHR, rr = q / (KC / B_PER), c = q % (KC / B_PER);
-            if (PLANES) {
-                const unsigned o = (unsigned)(rr * a.ldx + k0 + 8 * c) * 2u;
-                R.rbh[u] = rload16(rBh, o);
-                if (!SGN) R.rbl[u] = rload16(rBl, o);
-            } else {
-                R.rb[u] = rload16(rBh, (unsigned)(rr * a.ldx + k0 + 4 * c) * 4u);
-            }
+            if (PLANES)
+                R.rb[u] = rload16(rB, (unsigned)(rr * a.ldx + k0 + 8 * c) * 2u);
+            else
+                R.rb[u] = rload16(rB, (unsigned)(rr * a.ldx + k0 + 4 * c) * 4u);
         }
     };
     auto sstore = [&](auto S, int buf) {
@@ -554,8 +531,7 @@ __device__ __forceinline__ void split_tile(const StepArgs &a, int tile, unsigned
             if (B_LOADS % NTHR == 0 || q < B_LOADS) {
                 int rr = q / (KC / B_PER), c = q % (KC / B_PER);
                 if (PLANES) {
-                    *(u32x4 *)(Bh + (buf * NT + rr) * LD + 8 * c) = R.rbh[u];
-                    if (!SGN) *(u32x4 *)(Bl + (buf * NT + rr) * LD + 8 * c) = R.rbl[u];
+                    *(u32x4 *)(Bh + (buf * NT + rr) * LD + 8 * c) = R.rb[u];
                 } else {
                     u16x4 h, l;
                     const v4f xv = __builtin_bit_cast(v4f, R.rb[u]);
@@ -882,7 +858,7 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
     sb_update(a.c, F, x, y);
     a.Xn[o] = x;
     a.Y[o] = y;
-    if (a.Bn) write_plane1(a.bkind, x, a.Bn, a.bplane, o);
+    if (a.Bn) write_plane1(a.bkind, x, a.Bn, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -1007,22 +983,20 @@ typedef void (*StepKernel)(StepArgs);
 static const StepKernel kFp32Kernels[] = {SB_FP32_SHAPES(SB_FP32_PTR)};
 static const StepKernel kDsbFp8Kernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PTR)};
 static const StepKernel kDsbFp8PlKernels[] = {SB_FP8_SHAPES(SB_DSB_FP8_PL_PTR)};
-// Split kernels indexed [FP8 J][planes][sign B][shape].
+// Split kernels indexed [FP8 J][B source: hi/lo from X, sign from X, sign plane][shape].
 #define SB_SPLIT_PTR(F, P, G) \
     {SB_SPLIT_SHAPES(SB_SPLIT_PTR_1##F##P##G)}
 #define SB_SPLIT_PTR_1000(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, false, false>,
 #define SB_SPLIT_PTR_1001(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, false, true>,
-#define SB_SPLIT_PTR_1010(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, true, false>,
 #define SB_SPLIT_PTR_1011(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, false, true, true>,
 #define SB_SPLIT_PTR_1100(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, false, false>,
 #define SB_SPLIT_PTR_1101(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, false, true>,
-#define SB_SPLIT_PTR_1110(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true, false>,
 #define SB_SPLIT_PTR_1111(wm, wn, bm, bn) &k_step_split<wm, wn, bm, bn, true, true, true>,
 #define SB_COUNT_ONE(wm, wn, bm, bn) +1
 #define SB_COUNT_ONE1(rows) +1
-static const StepKernel kSplitKernels[2][2][2][0 SB_SPLIT_SHAPES(SB_COUNT_ONE)] = {
-    {{SB_SPLIT_PTR(0, 0, 0), SB_SPLIT_PTR(0, 0, 1)}, {SB_SPLIT_PTR(0, 1, 0), SB_SPLIT_PTR(0, 1, 1)}},
-    {{SB_SPLIT_PTR(1, 0, 0), SB_SPLIT_PTR(1, 0, 1)}, {SB_SPLIT_PTR(1, 1, 0), SB_SPLIT_PTR(1, 1, 1)}}};
+static const StepKernel kSplitKernels[2][3][0 SB_SPLIT_SHAPES(SB_COUNT_ONE)] = {
+    {SB_SPLIT_PTR(0, 0, 0), SB_SPLIT_PTR(0, 0, 1), SB_SPLIT_PTR(0, 1, 1)},
+    {SB_SPLIT_PTR(1, 0, 0), SB_SPLIT_PTR(1, 0, 1), SB_SPLIT_PTR(1, 1, 1)}};
 
 // Thin kernels indexed [J format: FP32, BF16, FP8][sign x][shape].
 #define SB_THIN_PTR_F32(rows) &k_step_thin<rows, 4, false>,
@@ -1065,7 +1039,7 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
     case FAM_FP32: k = kFp32Kernels[shape_idx]; break;
     case FAM_SPLIT_BF16:
     case FAM_SPLIT_FP8:
-        k = kSplitKernels[family == FAM_SPLIT_FP8][pl][a.c.variant == SB_DSB][shape_idx];
+        k = kSplitKernels[family == FAM_SPLIT_FP8][a.c.variant != SB_DSB ? 0 : pl ? 2 : 1][shape_idx];
         break;
     case FAM_DSB_FP8: k = pl ? kDsbFp8PlKernels[shape_idx] : kDsbFp8Kernels[shape_idx]; break;
     default: return hipErrorInvalidValue;
@@ -1266,7 +1240,6 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         a.Y = I.Y;
         a.B = I.P[k & 1];
         a.Bn = I.P[(k + 1) & 1];
-        a.bplane = I.bplane;
         a.bkind = I.bkind;
         a.ldj = I.ldx;
         a.ldx = I.ldx;
@@ -1290,15 +1263,15 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         case (FAM_FP32 * 2 + 1) * 8 + 3: fp32_tile<1, 4, 1, 1>(a, item.tile, (float *)smem_dyn); break;
         case (FAM_FP32 * 2 + 0) * 8 + 4:
         case (FAM_FP32 * 2 + 1) * 8 + 4: fp32_tile<2, 2, 1, 2>(a, item.tile, (float *)smem_dyn); break;
-        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, false, true, false>(a, item.tile, sm16); break;
-        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, false, true, false>(a, item.tile, sm16); break;
-        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, false, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, false, false, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, false, false, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_BF16 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, false, false, false>(a, item.tile, sm16); break;
         case (FAM_SPLIT_BF16 * 2 + 1) * 8 + 0: split_tile<2, 2, 1, 1, false, true, true>(a, item.tile, sm16); break;
         case (FAM_SPLIT_BF16 * 2 + 1) * 8 + 1: split_tile<2, 2, 2, 1, false, true, true>(a, item.tile, sm16); break;
         case (FAM_SPLIT_BF16 * 2 + 1) * 8 + 2: split_tile<4, 1, 1, 1, false, true, true>(a, item.tile, sm16); break;
-        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, true, true, false>(a, item.tile, sm16); break;
-        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, true, true, false>(a, item.tile, sm16); break;
-        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, true, true, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 0: split_tile<2, 2, 1, 1, true, false, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 1: split_tile<2, 2, 2, 1, true, false, false>(a, item.tile, sm16); break;
+        case (FAM_SPLIT_FP8 * 2 + 0) * 8 + 2: split_tile<4, 1, 1, 1, true, false, false>(a, item.tile, sm16); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 0: dsb_fp8_tile<2, 2, 1, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 1: dsb_fp8_tile<2, 2, 2, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 2: dsb_fp8_tile<4, 1, 1, 1, true>(a, item.tile, smem_dyn); break;
@@ -1412,11 +1385,11 @@ hipError_t launch_row_stats(const float *J, int N, int ld, double *row_ss, float
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_make_planes(int kind, const float *X, size_t n4, void *B, size_t bplane)
+__global__ __launch_bounds__(256) void k_make_planes(int kind, const float *X, size_t n4, void *B)
 {
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (size_t)gridDim.x * blockDim.x) {
         v4f xv = *(const v4f *)(X + 4 * q);
-        write_planes(kind, xv, B, bplane, 4 * q);
+        write_planes(kind, xv, B, 4 * q);
     }
 }
 
@@ -1425,7 +1398,7 @@ hipError_t launch_make_planes(int kind, const float *X, int R, int ldx, void *B,
     if (kind == BK_NONE || R <= 0) return hipSuccess;
     size_t n = (size_t)R * ldx, n4 = n / 4;
     size_t g = (n4 + 255) / 256;
-    hipLaunchKernelGGL(k_make_planes, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(256), 0, s, kind, X, n4, B, n);
+    hipLaunchKernelGGL(k_make_planes, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(256), 0, s, kind, X, n4, B);
     return hipGetLastError();
 }
 

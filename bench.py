@@ -200,11 +200,11 @@ def leg_batched(sb, ctx, n_inst, steps):
     leg["speedup_split_vs_fp32"] = leg["split"]["spin_updates_per_s"] / leg["fp32"]["spin_updates_per_s"]
     leg["mean_energy_gap_split_vs_fp32"] = (means["split"] - means["fp32"]) / abs(means["fp32"])
     # Algorithmic bytes per step: J once (FP8 split, FP32 plain) + the state of
-    # every spin-replica (X, Y read and written: 16 B; split bSB also reads and
-    # writes the hi/lo planes: 8 B more).
+    # every spin-replica (X, Y read and written: 16 B; the split form reads
+    # FP32 X and splits it into BF16 hi + lo while staging).
     jb = float((Ns.astype(np.float64) ** 2).sum())
     sr = float(Ns.sum()) * 64
-    for prec, bj, bs in (("split", 1.0, 24.0), ("fp32", 4.0, 16.0)):
+    for prec, bj, bs in (("split", 1.0, 16.0), ("fp32", 4.0, 16.0)):
         t = leg[prec]["avg_launch_us"] * 1e-6
         leg[prec]["alg_bytes_per_step"] = bj * jb + bs * sr
         leg[prec]["hbm_gbs_alg"] = (bj * jb + bs * sr) / t / 1e9

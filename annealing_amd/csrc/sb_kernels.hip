@@ -923,9 +923,10 @@ __global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
 #define SB_FP32_SHAPES(X) X(2, 2, 1, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1) \
     X(2, 4, 1, 1) X(4, 2, 1, 1) X(4, 2, 1, 2) X(2, 4, 2, 1) X(4, 2, 2, 1) X(2, 2, 4, 1) X(2, 2, 2, 4)
 #define SB_SPLIT_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
-    X(4, 2, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1)
+    X(4, 2, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1) X(4, 4, 2, 1)
 #define SB_FP8_SHAPES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(4, 1, 1, 1) \
-    X(4, 2, 2, 2) X(2, 4, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1)
+    X(4, 2, 2, 2) X(2, 4, 2, 2) X(4, 2, 2, 1) X(4, 2, 1, 1) X(2, 4, 1, 1) X(2, 2, 4, 2) X(2, 2, 4, 1) \
+    X(4, 4, 2, 1)
 
 // Thin shapes: spins per wave (4 waves per workgroup).
 #define SB_THIN_SHAPES(X) X(1) X(2) X(4) X(8)

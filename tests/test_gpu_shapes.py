@@ -11,7 +11,7 @@ from annealing_amd import sb
 pytestmark = pytest.mark.gpu
 
 # Compiled shapes per family (csrc/sb_kernels.hip: SB_*_SHAPES, SB_BATCH_*).
-SHAPES = {0: 13, 1: 10, 2: 10, 3: 11}
+SHAPES = {0: 13, 1: 11, 2: 11, 3: 12}
 BATCH_SHAPES = {0: 5, 1: 3, 2: 3, 3: 3}
 N, R, STEPS = 300, 40, 30
 

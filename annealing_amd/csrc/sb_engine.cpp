@@ -56,7 +56,7 @@ Engine::~Engine()
 size_t Engine::state_bytes(int N, int R)
 {
     const size_t e = (size_t)R * round_up(N, kSpinPad);
-    return e * sizeof(float) * 3 + e * 4 * 2;  // X[2], Y, planes[2] (<= 4 B/element)
+    return e * sizeof(float) * 3 + e * plane_bytes(BK_SIGN_BF16) * 2;  // X[2], Y, sign planes[2] (<= 2 B)
 }
 
 int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)

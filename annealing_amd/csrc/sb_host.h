@@ -17,7 +17,7 @@ struct sb_ctx {
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;
     size_t state_cap = 0;  // floats per buffer
-    // B-operand planes of X0 / X1 (4 bytes per state element: BF16 hi + lo).
+    // B-operand sign planes of X0 / X1 (dSB; at most 2 bytes per state element).
     unsigned char *P0 = nullptr, *P1 = nullptr;
     size_t plane_cap = 0;  // bytes per buffer
     double *H = nullptr, *part = nullptr;

@@ -209,6 +209,10 @@ def leg_batched(sb, ctx, n_inst, steps):
         leg[prec]["alg_bytes_per_step"] = bj * jb + bs * sr
         leg[prec]["hbm_gbs_alg"] = (bj * jb + bs * sr) / t / 1e9
         leg[prec]["hbm_gbs_j_only"] = bj * jb / t / 1e9
+        # HBM bytes per step from the committed counter runs (same workload)
+        tr = measured_traffic(f"batched ER512 R=64 {prec}") if n_inst == 512 else None
+        leg[prec]["traffic"] = tr
+        leg[prec]["hbm_gbs_measured"] = tr / t / 1e9 if tr else None
     for p in probs:
         p.close()
     return leg

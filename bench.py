@@ -367,8 +367,8 @@ def main() -> int:
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": 3}))
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(replicas shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU only; --gpus {args.gpus} is not supported (replicas and instances would "
+              "shard across GPUs with J replicated and no communication until the final gather)", file=sys.stderr)
         return 2
     if args.only_headline:
         args.no_cpu = args.no_precision = args.no_batched = args.no_serving = args.no_accuracy = True

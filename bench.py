@@ -234,6 +234,7 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
     lock = threading.Lock()
 
     def waiter():
+        last = time.perf_counter()
         for i in range(n_req):
             while tickets[i] is None:
                 time.sleep(0.001)
@@ -243,6 +244,9 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
                     lat[i] = r.total_ms
             except sb.SBError:
                 pass
+            if time.perf_counter() - last > 30.0:  # long windows: report progress as requests finish
+                last = time.perf_counter()
+                log(f"serving: {i + 1}/{n_req} requests retired")
         done_evt.set()
 
     th = threading.Thread(target=waiter, daemon=True)

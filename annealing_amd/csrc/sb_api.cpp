@@ -279,9 +279,17 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
     c->device = device;
     // Default stream priority only.
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->stream2) (void)hipStreamDestroy(c->stream2);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
         delete c;
         return sbh::hip_fail(e, "sb_ctx_create");
     }
@@ -330,10 +338,14 @@ extern "C" SB_API int sb_ctx_destroy(sb_ctx *ctx)
     }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
     free_ws(ctx);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->ev_fork);
+    (void)hipEventDestroy(ctx->ev_join);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return SB_OK;
 }
@@ -1026,6 +1038,20 @@ static int solve_sparse(sb_ctx *ctx, sb_problem *prob, const sb_params *p, doubl
     return SB_OK;
 }
 
+// dSB on the FP8 family is bound by its state traffic: each launch runs the
+// MFMA K loop and then the HBM-bound epilogue on every CU at once.  From this
+// many replicas up, the solve runs as two halves on two streams (each launch
+// then fills half the device), and the halves drift out of phase so one's
+// epilogue overlaps the other's K loop: 34.4 against 35.5 us per step for
+// K2000 at R=4096.  A replica's trajectory does not depend on R, so the
+// results are the same.
+constexpr int kTwoStreamMinR = 2048;
+
+static bool two_stream_solve(const sb_params *p, int family, int R)
+{
+    return p->variant == SB_DSB && family == FAM_DSB_FP8 && R >= kTwoStreamMinR;
+}
+
 int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)
 {
     if (prob->nnz > 0 && p->precision != SB_PREC_ORDERED) return solve_sparse(ctx, prob, p, c0, res);
@@ -1036,6 +1062,17 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     if (p->precision != SB_PREC_ORDERED) {
         rc = choose_kernel(ctx, prob, p->precision, p->variant, R, &family, &shape);
         if (rc) return rc;
+    }
+    // Replica halves [0, R1) on `stream` and [R1, R) on `stream2` (R1 a
+    // multiple of 128, the widest replica tile), each with its own kernel.
+    bool halves = p->precision != SB_PREC_ORDERED && two_stream_solve(p, family, R);
+    const int R1 = halves ? (R / 2) / 128 * 128 : R, R2 = R - R1;
+    int family1 = family, shape1 = shape, family2 = family, shape2 = shape;
+    if (halves) {
+        if ((rc = choose_kernel(ctx, prob, p->precision, p->variant, R1, &family1, &shape1))) return rc;
+        if ((rc = choose_kernel(ctx, prob, p->precision, p->variant, R2, &family2, &shape2))) return rc;
+        // The halves share the planes of one family; otherwise run whole.
+        halves = family1 == family && family2 == family;
     }
     rc = ensure_workspace(ctx, (size_t)R * ld, R, N);
     if (rc) return rc;
@@ -1059,6 +1096,12 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
         HIP_TRY(launch_make_planes(a.bkind, X, R, ld, Pn, s));
     }
     HIP_TRY(hipEventRecord(ctx->ev0, s));
+    if (halves) {
+        HIP_TRY(hipEventRecord(ctx->ev_fork, s));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    }
+    const size_t off2 = halves ? (size_t)R1 * ld : 0;  // first state element of the second half
+    const size_t poff2 = off2 * plane_bytes(a.bkind);
     for (int k = 0; k < p->steps; ++k) {
         sb_step_coeffs_for(p, c0, k, &a.c);
         a.X = X;
@@ -1068,11 +1111,33 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
             a.B = P;
             a.Bn = Pn;
         }
-        hipError_t e = p->precision == SB_PREC_ORDERED ? launch_step_ordered(a, s)
-                                                       : launch_step(family, shape, a, s);
+        hipError_t e;
+        if (p->precision == SB_PREC_ORDERED) {
+            e = launch_step_ordered(a, s);
+        } else if (!halves) {
+            e = launch_step(family, shape, a, s);
+        } else {
+            StepArgs a2 = a;
+            a.R = R1;
+            e = launch_step(family1, shape1, a, s);
+            a.R = R;
+            a2.R = R2;
+            a2.X = X + off2;
+            a2.Xn = Xn + off2;
+            a2.Y = Y + off2;
+            if (a2.bkind != BK_NONE) {
+                a2.B = P + poff2;
+                a2.Bn = Pn + poff2;
+            }
+            if (e == hipSuccess) e = launch_step(family2, shape2, a2, ctx->stream2);
+        }
         if (e != hipSuccess) return hip_fail(e, "launch_step");
         std::swap(X, Xn);
         std::swap(P, Pn);
+    }
+    if (halves) {
+        HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
+        HIP_TRY(hipStreamWaitEvent(s, ctx->ev_join, 0));
     }
     HIP_TRY(hipEventRecord(ctx->ev1, s));
     rc = read_results(ctx, prob, X, Y, R, res, s);

@@ -13,6 +13,10 @@ struct sb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Second stream for solves run as two concurrent replica halves, and the
+    // events that fork it from and join it back to `stream`.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::recursive_mutex mu;
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;

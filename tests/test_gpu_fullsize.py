@@ -97,3 +97,23 @@ def test_sk2000_r1024_fp32_readout(ctx, orc):
     sample = np.random.default_rng(2).choice(1024, 24, replace=False)
     check_readout(orc, J, sol, sample, exact=False)
     prob.close()
+
+
+@pytest.mark.parametrize("R", [2048, 2049, 3000])
+def test_dsb_replica_halves_match_single_stream_paths(ctx, orc, R):
+    # dSB on FP8 J from R = 2048 up runs as two replica halves on two streams
+    # (here 1024 + 1024, 1024 + 1025, 1408 + 1592); integer J makes every path
+    # exact, so it must equal the FP32 path bit for bit and the oracle on the
+    # replicas at both ends.
+    n = 300
+    J = sb.generate(sb.GEN_KN, n, 1.0, 41)
+    prob = ctx.problem(J)
+    kw = dict(variant="dsb", steps=30, replicas=R, init="uniform", seed=5, dt=0.5)
+    split = ctx.solve(prob, state=True, precision="split", **kw)
+    fp32 = ctx.solve(prob, state=True, precision="fp32", **kw)
+    assert np.array_equal(split.x, fp32.x) and np.array_equal(split.y, fp32.y)
+    assert np.array_equal(split.energy_i64, fp32.energy_i64)
+    xo, yo, _ = orc.solve_f32(sb.DSB, J, R, 30, init="uniform", seed=5, dt=0.5, nthreads=8)
+    for r in (0, R // 2 - 1, R // 2, R - 1):
+        assert np.array_equal(split.x[r], xo[r, :n]) and np.array_equal(split.y[r], yo[r, :n]), r
+    prob.close()

@@ -297,6 +297,8 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
     if (at && at[0] == '0') c->autotune = false;
     const char *vb = std::getenv("SB_VERBOSE");
     if (vb && vb[0] != '0') c->verbose = true;
+    // Debug override SB_HALVES_MIN_R=<R> (smallest R solved as two halves).
+    if (const char *hm = std::getenv("SB_HALVES_MIN_R")) c->halves_min_r = std::max(2, std::atoi(hm));
     *out = c;
     return SB_OK;
 }
@@ -1039,17 +1041,18 @@ static int solve_sparse(sb_ctx *ctx, sb_problem *prob, const sb_params *p, doubl
 }
 
 // dSB on the FP8 family is bound by its state traffic: each launch runs the
-// MFMA K loop and then the HBM-bound epilogue on every CU at once.  From this
-// many replicas up, the solve runs as two halves on two streams (each launch
-// then fills half the device), and the halves drift out of phase so one's
-// epilogue overlaps the other's K loop: 34.4 against 35.5 us per step for
-// K2000 at R=4096.  A replica's trajectory does not depend on R, so the
+// MFMA K loop and then the HBM-bound epilogue on every CU at once.  From
+// sb_ctx::halves_min_r replicas up, the solve runs as two halves on two
+// streams: the two kernels fill the device where one mid-sized grid leaves CUs
+// idle, and they drift out of phase so one's epilogue overlaps the other's K
+// loop.  K2000 per step: R=1536 15.9 against 17.5 us, R=2048 18.8 against
+// about 25 us, R=4096 34.4 against 35.5 us; at R <= 1024 (N = 1000 and 2000)
+// one launch is faster.  A replica's trajectory does not depend on R, so the
 // results are the same.
-constexpr int kTwoStreamMinR = 2048;
 
-static bool two_stream_solve(const sb_params *p, int family, int R)
+static bool two_stream_solve(const sb_ctx *ctx, const sb_params *p, int family, int R)
 {
-    return p->variant == SB_DSB && family == FAM_DSB_FP8 && R >= kTwoStreamMinR;
+    return p->variant == SB_DSB && family == FAM_DSB_FP8 && R >= ctx->halves_min_r;
 }
 
 int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)
@@ -1065,7 +1068,7 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     }
     // Replica halves [0, R1) on `stream` and [R1, R) on `stream2` (R1 a
     // multiple of 128, the widest replica tile), each with its own kernel.
-    bool halves = p->precision != SB_PREC_ORDERED && two_stream_solve(p, family, R);
+    bool halves = p->precision != SB_PREC_ORDERED && two_stream_solve(ctx, p, family, R);
     const int R1 = halves ? (R / 2) / 128 * 128 : R, R2 = R - R1;
     int family1 = family, shape1 = shape, family2 = family, shape2 = shape;
     if (halves) {

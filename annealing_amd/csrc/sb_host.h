@@ -17,6 +17,7 @@ struct sb_ctx {
     // events that fork it from and join it back to `stream`.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int halves_min_r = 1536;  // smallest R of a dSB (FP8 family) solve run as two halves
     std::recursive_mutex mu;
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;

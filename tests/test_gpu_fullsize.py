@@ -99,10 +99,10 @@ def test_sk2000_r1024_fp32_readout(ctx, orc):
     prob.close()
 
 
-@pytest.mark.parametrize("R", [2048, 2049, 3000])
+@pytest.mark.parametrize("R", [1536, 2048, 2049, 3000])
 def test_dsb_replica_halves_match_single_stream_paths(ctx, orc, R):
-    # dSB on FP8 J from R = 2048 up runs as two replica halves on two streams
-    # (here 1024 + 1024, 1024 + 1025, 1408 + 1592); integer J makes every path
+    # dSB on FP8 J from R = 1536 up runs as two replica halves on two streams
+    # (here 768 + 768, 1024 + 1024, 1024 + 1025, 1408 + 1592); integer J makes every path
     # exact, so it must equal the FP32 path bit for bit and the oracle on the
     # replicas at both ends.
     n = 300

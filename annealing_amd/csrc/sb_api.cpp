@@ -297,8 +297,9 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
     if (at && at[0] == '0') c->autotune = false;
     const char *vb = std::getenv("SB_VERBOSE");
     if (vb && vb[0] != '0') c->verbose = true;
-    // Debug override SB_HALVES_MIN_R=<R> (smallest R solved as two halves).
-    if (const char *hm = std::getenv("SB_HALVES_MIN_R")) c->halves_min_r = std::max(2, std::atoi(hm));
+    // Debug override SB_HALVES_MIN_R=<R> for both families (a large value turns halves off).
+    if (const char *hm = std::getenv("SB_HALVES_MIN_R"))
+        c->halves_min_r_fp8 = c->halves_min_r_fp32 = std::max(2, std::atoi(hm));
     *out = c;
     return SB_OK;
 }
@@ -1040,19 +1041,23 @@ static int solve_sparse(sb_ctx *ctx, sb_problem *prob, const sb_params *p, doubl
     return SB_OK;
 }
 
-// dSB on the FP8 family is bound by its state traffic: each launch runs the
-// MFMA K loop and then the HBM-bound epilogue on every CU at once.  From
-// sb_ctx::halves_min_r replicas up, the solve runs as two halves on two
-// streams: the two kernels fill the device where one mid-sized grid leaves CUs
-// idle, and they drift out of phase so one's epilogue overlaps the other's K
-// loop.  K2000 per step: R=1536 15.9 against 17.5 us, R=2048 18.8 against
-// about 25 us, R=4096 34.4 against 35.5 us; at R <= 1024 (N = 1000 and 2000)
-// one launch is faster.  A replica's trajectory does not depend on R, so the
-// results are the same.
-
+// Replica halves.  Every launch of a step runs the K loop and then the
+// epilogue on every CU at once.  From a family's threshold up, the solve runs
+// as two halves on two streams: the two kernels fill the device where one
+// mid-sized grid leaves CUs idle, and they drift out of phase so one's
+// epilogue overlaps the other's K loop.  Measured per step:
+//   dSB, FP8 family (state-bound), K2000: R=1536 15.9 against 17.5 us,
+//     R=2048 18.8 against about 25 us, R=4096 34.4 against 35.5 us; at
+//     R <= 1024 (N = 1000 and 2000) one launch is faster;
+//   FP32 family, SK N=2000: R=1024 63.6 against 66.5-69.2 us, R=2048 121.5
+//     against 130 us, R=4096 245.7 against 249.8 us;
+//   split bSB (MFMA-bound): no gain up to R=3000, slower at R=4096.
+// A replica's trajectory does not depend on R, so the results are the same.
 static bool two_stream_solve(const sb_ctx *ctx, const sb_params *p, int family, int R)
 {
-    return p->variant == SB_DSB && family == FAM_DSB_FP8 && R >= ctx->halves_min_r;
+    if (p->variant == SB_DSB && family == FAM_DSB_FP8) return R >= ctx->halves_min_r_fp8;
+    if (family == FAM_FP32) return R >= ctx->halves_min_r_fp32;
+    return false;
 }
 
 int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)

@@ -17,7 +17,9 @@ struct sb_ctx {
     // events that fork it from and join it back to `stream`.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    int halves_min_r = 1536;  // smallest R of a dSB (FP8 family) solve run as two halves
+    // Smallest R of a solve run as two replica halves: dSB on the FP8 family,
+    // and the FP32 family (any variant).  Split bSB/aSB always run whole.
+    int halves_min_r_fp8 = 1536, halves_min_r_fp32 = 1024;
     std::recursive_mutex mu;
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;

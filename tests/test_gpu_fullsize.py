@@ -117,3 +117,22 @@ def test_dsb_replica_halves_match_single_stream_paths(ctx, orc, R):
     for r in (0, R // 2 - 1, R // 2, R - 1):
         assert np.array_equal(split.x[r], xo[r, :n]) and np.array_equal(split.y[r], yo[r, :n]), r
     prob.close()
+
+
+@pytest.mark.parametrize("variant,precision,J_kind,R", [("bsb", "fp32", "sk", 1024), ("dsb", "fp32", "sk", 1024),
+                                                        ("asb", "fp32", "sk", 1024), ("dsb", "split", "kn", 1536)])
+def test_replica_halves_equal_single_launch(ctx, variant, precision, J_kind, R):
+    # From R = 1024 (FP32 family) and R = 1536 (dSB on FP8 J) a solve runs as two
+    # replica halves on two streams; one replica fewer runs as a single launch.
+    # The first R - 1 replicas must be bit-identical (a replica's trajectory
+    # does not depend on R or on the tile shape).
+    n = 700
+    J = sb.generate(sb.GEN_SK if J_kind == "sk" else sb.GEN_KN, n, 1.0, 43)
+    prob = ctx.problem(J)
+    kw = dict(variant=variant, precision=precision, steps=40, init="uniform", seed=6, dt=0.5 if variant != "asb" else 1.0)
+    halves = ctx.solve(prob, state=True, replicas=R, **kw)
+    single = ctx.solve(prob, state=True, replicas=R - 1, **kw)
+    assert np.array_equal(halves.x[:R - 1], single.x, equal_nan=True)
+    assert np.array_equal(halves.y[:R - 1], single.y, equal_nan=True)
+    assert np.array_equal(halves.energy[:R - 1], single.energy)
+    prob.close()

@@ -1049,8 +1049,9 @@ static int solve_sparse(sb_ctx *ctx, sb_problem *prob, const sb_params *p, doubl
 //   dSB, FP8 family (state-bound), K2000: R=1536 15.9 against 17.5 us,
 //     R=2048 18.8 against about 25 us, R=4096 34.4 against 35.5 us; at
 //     R <= 1024 (N = 1000 and 2000) one launch is faster;
-//   FP32 family, SK N=2000: R=1024 63.6 against 66.5-69.2 us, R=2048 121.5
-//     against 130 us, R=4096 245.7 against 249.8 us;
+//   FP32 family, SK N=2000: R=768 59.2 against 65.4 us, R=1024 63.6 against
+//     66.5-69.2 us, R=2048 121.5 against 130 us, R=4096 245.7 against
+//     249.8 us; at R <= 512 one launch is faster (39.7 against 47.9 us);
 //   split bSB (MFMA-bound): no gain up to R=3000, slower at R=4096.
 // A replica's trajectory does not depend on R, so the results are the same.
 static bool two_stream_solve(const sb_ctx *ctx, const sb_params *p, int family, int R)

@@ -19,7 +19,7 @@ struct sb_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // Smallest R of a solve run as two replica halves: dSB on the FP8 family,
     // and the FP32 family (any variant).  Split bSB/aSB always run whole.
-    int halves_min_r_fp8 = 1536, halves_min_r_fp32 = 1024;
+    int halves_min_r_fp8 = 1536, halves_min_r_fp32 = 768;
     std::recursive_mutex mu;
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;

@@ -374,6 +374,10 @@ static int grow(T **p, size_t *cap, size_t need)
 int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N)
 {
     int rc;
+    // Solves run replica halves on stream2 as well; both streams must be idle
+    // before a buffer is freed and reallocated.
+    if (ctx->state_cap < state_elems || ctx->plane_cap < plane_bytes(BK_SIGN_BF16) * state_elems)
+        (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->state_cap < state_elems) {
         (void)hipStreamSynchronize(ctx->stream);
         size_t c0 = ctx->state_cap, c1 = ctx->state_cap, c2 = ctx->state_cap;
@@ -390,7 +394,7 @@ int ensure_workspace(sb_ctx *ctx, size_t state_elems, int R, int N)
         if ((rc = grow(&ctx->P1, &c1, plane_elems))) return rc;
         ctx->plane_cap = plane_elems;
     }
-    if ((rc = grow(&ctx->H, &ctx->H_cap, (size_t)std::max(R, 1)))) return rc;
+    if ((rc = grow(&ctx->H, &ctx->H_cap, 3 * (size_t)std::max(R, 1)))) return rc;
     if ((rc = grow(&ctx->part, &ctx->part_cap, std::max<size_t>(energy_partial_elems(N, R), 1)))) return rc;
     if ((rc = grow(&ctx->S, &ctx->S_cap, std::max<size_t>((size_t)R * N, 1)))) return rc;
     if ((rc = grow(&ctx->tmp, &ctx->tmp_cap, std::max<size_t>((size_t)R * N, 1)))) return rc;
@@ -430,6 +434,8 @@ extern "C" SB_API int sb_problem_create(sb_ctx *ctx, int32_t N, const float *J, 
     p->integer = integer != 0;
     p->sigma = sigma;
     p->max_abs = mx;
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j) p->sum_upper += (double)J[(size_t)i * ldj + j];
     const size_t elems = (size_t)N * p->ld;
     std::vector<float> host(elems, 0.0f);
     for (int i = 0; i < N; ++i) std::memcpy(&host[(size_t)i * p->ld], J + (size_t)i * ldj, sizeof(float) * N);
@@ -492,6 +498,7 @@ extern "C" SB_API int sb_problem_destroy(sb_problem *p)
     std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
+    (void)hipStreamSynchronize(p->ctx->stream2);
     (void)hipFree(p->J32);
     (void)hipFree(p->Jbf16);
     (void)hipFree(p->Jfp8);
@@ -630,17 +637,17 @@ extern "C" SB_API int sb_problem_generate(sb_ctx *ctx, int32_t kind, int32_t N, 
     double *row_ss = nullptr;
     float *row_max = nullptr;
     int *row_int = nullptr;
-    std::vector<double> ss(N);
+    std::vector<double> ss(2 * (size_t)N);
     std::vector<float> mx(N);
     std::vector<int> in(N);
     hipStream_t s = ctx->stream;
     hipError_t e = hipMalloc((void **)&pr->J32, elems * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void **)&row_ss, N * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&row_ss, 2 * (size_t)N * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void **)&row_max, N * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&row_int, N * sizeof(int));
     if (e == hipSuccess) e = launch_generate(kind, N, p, seed, pr->J32, pr->ld, s);
     if (e == hipSuccess) e = launch_row_stats(pr->J32, N, pr->ld, row_ss, row_max, row_int, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ss.data(), row_ss, N * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ss.data(), row_ss, 2 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(mx.data(), row_max, N * sizeof(float), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(in.data(), row_int, N * sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -648,6 +655,7 @@ extern "C" SB_API int sb_problem_generate(sb_ctx *ctx, int32_t kind, int32_t N, 
     bool integer = true;
     for (int i = 0; i < N; ++i) {
         tot += ss[i];
+        pr->sum_upper += ss[(size_t)N + i];
         m = std::max(m, (double)mx[i]);
         integer = integer && in[i] != 0;
     }
@@ -941,13 +949,27 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
 // Energies are exact integers (int64) when J and the external field are.
 static bool exact_energy(const sb_problem *prob) { return prob->integer && (!prob->h || prob->h_integer); }
 
+// Integer-valued J with |J_ij| <= 127: the coupling energy runs as the exact
+// integer reduction (int8 MFMA); other J in FP64.
+static bool integer_energy(const sb_problem *prob) { return prob->integer && prob->max_abs <= 127.0; }
+
 int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const float *Y, int R, sb_result *res,
                  hipStream_t s)
 {
     const int N = prob->N, ld = prob->ld;
-    HIP_TRY(launch_energy(prob->J32, ld, prob->h, X, ld, N, R, ctx->part, ctx->H, s));
-    std::vector<double> H(R);
-    HIP_TRY(hipMemcpyAsync(H.data(), ctx->H, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+    // ctx->H: [0, R) coupling part (FP64), [R, 2R) field part, [2R, 3R) the
+    // coupling part as int64 (integer form only).
+    double *Hc = ctx->H, *Hf = ctx->H + R;
+    long long *Hi = (long long *)(ctx->H + 2 * (size_t)R);
+    const bool ie = integer_energy(prob);
+    if (ie) HIP_TRY(launch_energy_i8(prob->J32, ld, X, ld, N, R, (long long *)ctx->part, Hi, Hc, s));
+    else HIP_TRY(launch_energy(prob->J32, ld, X, ld, N, R, ctx->part, Hc, s));
+    if (prob->h) HIP_TRY(launch_energy_field(prob->h, X, ld, N, R, Hf, s));
+    std::vector<double> hc(R), hf(R, 0.0);
+    std::vector<long long> hi(ie ? R : 0);
+    HIP_TRY(hipMemcpyAsync(hc.data(), Hc, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+    if (prob->h) HIP_TRY(hipMemcpyAsync(hf.data(), Hf, sizeof(double) * R, hipMemcpyDeviceToHost, s));
+    if (ie) HIP_TRY(hipMemcpyAsync(hi.data(), Hi, sizeof(long long) * R, hipMemcpyDeviceToHost, s));
     if (res->spins) {
         HIP_TRY(launch_spins(X, ld, N, R, ctx->S, s));
         HIP_TRY(hipMemcpyAsync(res->spins, ctx->S, (size_t)R * N, hipMemcpyDeviceToHost, s));
@@ -962,14 +984,23 @@ int read_results(sb_ctx *ctx, const sb_problem *prob, const float *X, const floa
         HIP_TRY(hipMemcpyAsync(res->y_final, ctx->tmp, sizeof(float) * (size_t)R * N, hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
+    const bool exact = exact_energy(prob);
     int best = 0;
+    std::vector<double> H(R);
     for (int r = 0; r < R; ++r) {
+        H[r] = prob->h ? hc[r] + hf[r] : hc[r];
         if (res->energy_f64) res->energy_f64[r] = H[r];
-        if (res->energy_i64) res->energy_i64[r] = exact_energy(prob) ? (int64_t)std::llround(H[r]) : 0;
+        if (res->energy_i64)
+            res->energy_i64[r] = !exact ? 0
+                                 : ie ? (int64_t)hi[r] + (prob->h ? (int64_t)std::llround(hf[r]) : 0)
+                                      : (int64_t)std::llround(H[r]);
+        // Max-cut mapping W = -J: cut = (sum_{i<j} W_ij - sum_{i<j} W_ij s_i s_j) / 2
+        //                             = (-sum_{i<j} J_ij - Hc) / 2.
+        if (res->cut) res->cut[r] = (-prob->sum_upper - hc[r]) * 0.5;
         if (H[r] < H[best]) best = r;
     }
     res->best_replica = best;
-    res->energy_exact = exact_energy(prob) ? 1 : 0;
+    res->energy_exact = exact ? 1 : 0;
     return SB_OK;
 }
 
@@ -1140,7 +1171,12 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
             }
             if (e == hipSuccess) e = launch_step(family2, shape2, a2, ctx->stream2);
         }
-        if (e != hipSuccess) return hip_fail(e, "launch_step");
+        if (e != hipSuccess) {
+            // The other half may still have kernels in flight on the shared
+            // workspace: drain stream2 before the workspace can be reused.
+            if (halves) (void)hipStreamSynchronize(ctx->stream2);
+            return hip_fail(e, "launch_step");
+        }
         std::swap(X, Xn);
         std::swap(P, Pn);
     }
@@ -1233,13 +1269,24 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
     int rc = sbh::ensure_workspace(ctx, (size_t)R * ld, R, N);
     if (rc) return rc;
     hipStream_t s = ctx->stream;
-    // X into the padded workspace, Y = 0; one step with alpha = gamma = 0,
-    // beta = 1 leaves y' = fmaf(1, F, fmaf(-0, x, 0)) = F (bSB/dSB, |x| <= 1
-    // so no wall) or y' = fmaf(1, F, 0) = F (aSB without local sub-steps).
+    // X into the padded workspace, Y = 0, then one step that returns the field
+    // in Y.  bSB, aSB and aSB-m1 run aSB's kick with beta = 1 and no local
+    // sub-steps: y' = fmaf(1, F, 0) = F, with no wall whatever |x| is.  dSB
+    // runs its own step (the sign of x feeds the product) with
+    // alpha = gamma = 0: y' = fmaf(1, F, fmaf(-0, x, 0)) = F, x' = x; its
+    // amplitudes are clamped to [-1, 1] first (same signs, so the same field)
+    // so the wall never resets y'.
+    std::vector<float> Xc;
+    const float *Xsrc = X;
+    if (variant == SB_DSB) {
+        Xc.assign(X, X + (size_t)R * N);
+        for (float &v : Xc) v = std::max(-1.0f, std::min(1.0f, v));
+        Xsrc = Xc.data();
+    }
     HIP_TRY(hipMemsetAsync(ctx->X0, 0, (size_t)R * ld * sizeof(float), s));
     HIP_TRY(hipMemsetAsync(ctx->X1, 0, (size_t)R * ld * sizeof(float), s));
     HIP_TRY(hipMemsetAsync(ctx->Y, 0, (size_t)R * ld * sizeof(float), s));
-    HIP_TRY(hipMemcpy2DAsync(ctx->X0, (size_t)ld * sizeof(float), X, (size_t)N * sizeof(float),
+    HIP_TRY(hipMemcpy2DAsync(ctx->X0, (size_t)ld * sizeof(float), Xsrc, (size_t)N * sizeof(float),
                              (size_t)N * sizeof(float), (size_t)R, hipMemcpyHostToDevice, s));
     StepArgs a{};
     a.X = ctx->X0;
@@ -1250,8 +1297,8 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
     a.N = N;
     a.R = R;
     a.h = prob->h;
-    // aSB-m1 has the same field J.x; aSB's kick with beta = 1 returns it.
-    a.c.variant = variant == SB_ASB_M1 ? SB_ASB : variant;
+    // J.x for bSB, aSB, aSB-m1 through aSB's kick; J.sign(x) through dSB's step.
+    a.c.variant = variant == SB_DSB ? SB_DSB : SB_ASB;
     a.c.beta = 1.0f;
     a.c.do_local = 0;
     hipError_t e;
@@ -1280,7 +1327,7 @@ extern "C" SB_API int sb_field(sb_ctx *ctx, sb_problem *prob, int32_t precision,
         // choose_kernel may have used the workspace: restore X and Y.
         HIP_TRY(hipMemsetAsync(ctx->X0, 0, (size_t)R * ld * sizeof(float), s));
         HIP_TRY(hipMemsetAsync(ctx->Y, 0, (size_t)R * ld * sizeof(float), s));
-        HIP_TRY(hipMemcpy2DAsync(ctx->X0, (size_t)ld * sizeof(float), X, (size_t)N * sizeof(float),
+        HIP_TRY(hipMemcpy2DAsync(ctx->X0, (size_t)ld * sizeof(float), Xsrc, (size_t)N * sizeof(float),
                                  (size_t)N * sizeof(float), (size_t)R, hipMemcpyHostToDevice, s));
         a.J = sbh::family_J(family, prob);
         e = launch_step(family, shape, a, s);  // B staged from X (no planes)
@@ -1326,6 +1373,6 @@ extern "C" SB_API int sb_energy(sb_ctx *ctx, const float *dJ, int32_t N, int32_t
     size_t need = energy_partial_elems(N, R);
     int rc = sbh::grow(&ctx->part, &ctx->part_cap, need);
     if (rc) return rc;
-    HIP_TRY(launch_energy(dJ, ldj, nullptr, dX, ldx, N, R, ctx->part, dH, s));
+    HIP_TRY(launch_energy(dJ, ldj, dX, ldx, N, R, ctx->part, dH, s));
     return SB_OK;
 }

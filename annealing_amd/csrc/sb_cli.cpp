@@ -5,6 +5,7 @@
 // defines become the options below, with the reference's values as defaults.
 #include <cerrno>
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,6 +35,8 @@ static void usage()
         "  --sparse                  CSR field (low-density instances; same sums as ordered)\n"
         "  --output FILE             energies (default output.txt)\n"
         "  --spins FILE              also write spins, one replica per line\n"
+        "  --cut FILE                also write the cut value of the max-cut mapping W = -J,\n"
+        "                            one replica per line\n"
         "  --device D                GPU ordinal (default 0)\n");
 }
 
@@ -66,7 +69,7 @@ int main(int argc, char **argv)
     sb_params p;
     sb_default_params(&p);
     const char *instance = nullptr;
-    std::string out_path = "output.txt", spins_path;
+    std::string out_path = "output.txt", spins_path, cut_path;
     int one_based = 0, device = 0, sparse = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -114,6 +117,7 @@ int main(int argc, char **argv)
         else if (a == "--sparse") sparse = 1;
         else if (a == "--output") out_path = need("--output");
         else if (a == "--spins") spins_path = need("--spins");
+        else if (a == "--cut") cut_path = need("--cut");
         else if (a == "--help" || a == "-h") { usage(); return 0; }
         else if (a.size() > 1 && a[0] == '-' && a[1] == '-') {
             std::fprintf(stderr, "bad option or value: %s\n", a.c_str());
@@ -148,11 +152,13 @@ int main(int argc, char **argv)
     std::vector<int64_t> Hi(R);
     std::vector<double> Hd(R);
     std::vector<int8_t> S(spins_path.empty() ? 0 : (size_t)R * N);
+    std::vector<double> cut(R);
     sb_result res;
     std::memset(&res, 0, sizeof(res));
     res.energy_i64 = Hi.data();
     res.energy_f64 = Hd.data();
     res.spins = S.empty() ? nullptr : S.data();
+    res.cut = cut.data();
     rc = sb_solve(ctx, prob, &p, &res);
     if (rc) {
         std::fprintf(stderr, "sb_solve: %s (%s)\n", sb_status_string(rc), sb_last_error_message());
@@ -167,11 +173,24 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "cannot write %s\n", out_path.c_str());
         return 1;
     }
+    // Integers when the energies are exact (integer J and field), else FP64.
     for (int r = 0; r < R; ++r) {
-        if (integer) std::fprintf(f, "%" PRId64 "\n", Hi[r]);
+        if (res.energy_exact) std::fprintf(f, "%" PRId64 "\n", Hi[r]);
         else std::fprintf(f, "%.17g\n", Hd[r]);
     }
     std::fclose(f);
+    if (!cut_path.empty()) {
+        FILE *c = std::fopen(cut_path.c_str(), "w");
+        if (!c) {
+            std::fprintf(stderr, "cannot write %s\n", cut_path.c_str());
+            return 1;
+        }
+        for (int r = 0; r < R; ++r) {
+            if (res.energy_exact) std::fprintf(c, "%" PRId64 "\n", (int64_t)std::llround(cut[r]));
+            else std::fprintf(c, "%.17g\n", cut[r]);
+        }
+        std::fclose(c);
+    }
     if (!spins_path.empty()) {
         FILE *g = std::fopen(spins_path.c_str(), "w");
         if (!g) {

@@ -27,6 +27,8 @@ struct sb_ctx {
     // B-operand sign planes of X0 / X1 (dSB; at most 2 bytes per state element).
     unsigned char *P0 = nullptr, *P1 = nullptr;
     size_t plane_cap = 0;  // bytes per buffer
+    // Energies: H holds 3 R doubles (coupling part, field part, int64 exact
+    // coupling part); part holds the per-row-tile partials (FP64 or int64).
     double *H = nullptr, *part = nullptr;
     size_t H_cap = 0, part_cap = 0;
     int8_t *S = nullptr;
@@ -48,6 +50,7 @@ struct sb_problem {
     int N = 0, ld = 0;
     bool integer = false;
     double sigma = 0.0, max_abs = 0.0;
+    double sum_upper = 0.0;   // sum_{i<j} J_ij (FP64; exact for integer J)
     float *J32 = nullptr;
     uint16_t *Jbf16 = nullptr;
     uint8_t *Jfp8 = nullptr;  // only when integer and |J| <= 16

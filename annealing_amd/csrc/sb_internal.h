@@ -127,9 +127,19 @@ hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X,
                        float *X2, hipStream_t s);
 hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, float *X, float *Y,
                             hipStream_t s);
-// H_r = -sum_{i<j} J_ij s_i s_j (- sum_i h_i s_i when h is not null).
-hipError_t launch_energy(const float *J, int ldj, const float *h, const float *X, int ldx, int N, int R,
-                         double *partial, double *H, hipStream_t s);
+// Coupling energy Hc_r = -sum_{i<j} J_ij s_i s_j of s = sign(X) in FP64
+// (exact for integer J below 2^53); `partial` holds energy_partial_elems
+// doubles.
+hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N, int R, double *partial,
+                         double *H, hipStream_t s);
+// The same for integer-valued J with |J_ij| <= 127 as an exact integer
+// reduction (int8 MFMA, int32 accumulators, int64 sums): Hi (int64) and H
+// (its FP64 value); `partial` holds energy_partial_elems int64.  ldj and ldx
+// are multiples of 64.
+hipError_t launch_energy_i8(const float *J, int ldj, const float *X, int ldx, int N, int R, long long *partial,
+                            long long *Hi, double *H, hipStream_t s);
+// Field part Hf_r = -sum_i h_i s_i.
+hipError_t launch_energy_field(const float *h, const float *X, int ldx, int N, int R, double *Hf, hipStream_t s);
 size_t energy_partial_elems(int N, int R);
 hipError_t launch_spins(const float *X, int ldx, int N, int R, int8_t *S, hipStream_t s);
 hipError_t launch_compact(const float *X, int ldx, int N, int R, float *out, hipStream_t s);
@@ -225,7 +235,8 @@ SB_HD inline float gen_coupling(int kind, int N, double p, uint64_t seed, uint32
 }
 
 // Device generation of a dense J [N][ld] (zero diagonal and padding), and
-// per-row statistics: sum of squares (FP64), max |J|, integer flag.
+// per-row statistics: sum of squares and sum right of the diagonal (FP64,
+// row_ss holds 2 N doubles), max |J|, integer flag.
 hipError_t launch_generate(int kind, int N, double p, uint64_t seed, float *J, int ld, hipStream_t s);
 hipError_t launch_row_stats(const float *J, int N, int ld, double *row_ss, float *row_max, int *row_int,
                             hipStream_t s);

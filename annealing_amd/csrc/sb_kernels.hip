@@ -221,31 +221,35 @@ hipError_t launch_generate(int kind, int N, double p, uint64_t seed, float *J, i
     return hipGetLastError();
 }
 
-// One workgroup per row: sum of squares (FP64, fixed tree), max |J|, and
-// whether every entry is an integer.
+// One workgroup per row: sum of squares and sum of the entries right of the
+// diagonal (FP64, fixed tree), max |J|, and whether every entry is an
+// integer.  row_ss holds 2 N doubles: the sums of squares, then the upper sums.
 __global__ __launch_bounds__(256) void k_row_stats(const float *J, int N, int ld, double *row_ss, float *row_max,
                                                    int *row_int)
 {
-    __shared__ double ss[256];
+    __shared__ double ss[256], su[256];
     __shared__ float mx[256];
     __shared__ int in[256];
     const int i = blockIdx.x, t = threadIdx.x;
-    double s = 0.0;
+    double s = 0.0, u = 0.0;
     float m = 0.0f;
     int integ = 1;
     for (int j = t; j < N; j += 256) {
         float v = J[(size_t)i * ld + j];
         s += (double)v * (double)v;
+        if (j > i) u += (double)v;
         m = fmaxf(m, fabsf(v));
         if (v != floorf(v)) integ = 0;
     }
     ss[t] = s;
+    su[t] = u;
     mx[t] = m;
     in[t] = integ;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (t < w) {
             ss[t] += ss[t + w];
+            su[t] += su[t + w];
             mx[t] = fmaxf(mx[t], mx[t + w]);
             in[t] &= in[t + w];
         }
@@ -253,6 +257,7 @@ __global__ __launch_bounds__(256) void k_row_stats(const float *J, int N, int ld
     }
     if (t == 0) {
         row_ss[i] = ss[0];
+        row_ss[N + i] = su[0];
         row_max[i] = mx[0];
         row_int[i] = in[0];
     }
@@ -398,10 +403,14 @@ hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, floa
 }
 
 // ---------------------------------------------------------------------------
-// Energy: H_r = -1/2 sum_i s_i G_i with G = J s, in FP64.  Tile 64 spins x 64
-// replicas per workgroup; every product is +-J_ij (exact), so for integer J
-// all partial sums are exact integers below 2^53 and the result is exact.
-// Deterministic: fixed per-thread order and a fixed reduction tree.
+// Energies.  The coupling part Hc_r = -sum_{i<j} J_ij s_i s_j = -1/2 sum_i s_i G_i
+// with G = J s, s = x > 0 ? +1 : -1, and the field part -sum_i h_i s_i are
+// computed separately (the cut value of the max-cut mapping needs Hc alone).
+//
+// FP64 form (any J): tile 64 spins x 64 replicas per workgroup; every
+// product is +-J_ij (exact), so for integer J all partial sums are exact
+// integers below 2^53.  Deterministic: fixed per-thread order and a fixed
+// reduction tree.
 
 constexpr int EN_T = 64, EN_KC = 32;
 
@@ -473,9 +482,109 @@ __global__ __launch_bounds__(256) void k_energy_final(const double *part, int ti
     H[r] = -0.5 * s;
 }
 
-// The field term -sum_i h_i s_i per replica: one workgroup per replica, a
-// fixed per-thread order and a fixed tree (deterministic; exact for integer h).
-__global__ __launch_bounds__(256) void k_energy_field(const float *h, const float *X, int ldx, int N, double *H)
+// Integer form, for integer-valued J with |J_ij| <= 127: G = J s on
+// v_mfma_i32_32x32x32_i8 (int8 J x int8 signs, int32 accumulators, exact
+// for |G_i| <= 127 N), then s_i G_i summed in int64 over the tile's spins
+// and, in k_energy_final_i64, over the row tiles.  Tile 64 spins x 64
+// replicas, four waves of one 32 x 32 block; the signs are the MFMA's first
+// operand, so accumulator v of lane l is replica 8 (v / 4) + 4 (l / 32) + v % 4
+// and spin l % 32 of the wave's block.  A and B take the same k positions,
+// so the contraction is exact whatever the hardware's k numbering.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+constexpr int EI_T = 64, EI_KC = 64, EI_LD = EI_KC + 16;
+
+__device__ __forceinline__ unsigned int pack_i8x4(int a, int b, int c, int d)
+{
+    return (unsigned)(a & 0xff) | ((unsigned)(b & 0xff) << 8) | ((unsigned)(c & 0xff) << 16) |
+           ((unsigned)(d & 0xff) << 24);
+}
+
+__global__ __launch_bounds__(256) void k_energy_i8(const float *J, int ldj, const float *X, int ldx, int N,
+                                                   int R, long long *part)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char Js[EI_T][EI_LD];
+    __shared__ __attribute__((aligned(16))) unsigned char Ss[EI_T][EI_LD];
+    __shared__ long long red[2][EI_T];
+    const int tiles_m = (N + EI_T - 1) / EI_T;
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int row0 = tm * EI_T, col0 = tn * EI_T;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    // Staging: thread t converts 16 consecutive k (at 16 (t % 4)) of J row
+    // t / 4 and of replica t / 4; rows past N and replicas past R are zeros.
+    const int sr = tid >> 2, sc = 16 * (tid & 3);
+    const bool jrow = row0 + sr < N, srep = col0 + sr < R;
+    const float *Jp = J + (size_t)(jrow ? row0 + sr : 0) * ldj + sc;
+    const float *Xp = X + (size_t)(srep ? col0 + sr : 0) * ldx + sc;
+    v16i acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0;
+    // k < round_up(N, 64) <= ldj, ldx; J is zero in the padding columns.
+    const int Kp = round_up_dev(N, EI_KC);
+    for (int k0 = 0; k0 < Kp; k0 += EI_KC) {
+        u32x4 jw = {0u, 0u, 0u, 0u}, sw = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (jrow) {
+                const v4f v = *(const v4f *)(Jp + k0 + 4 * g);
+                jw[g] = pack_i8x4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+            }
+            if (srep) {
+                const v4f v = *(const v4f *)(Xp + k0 + 4 * g);
+                sw[g] = pack_i8x4(v[0] > 0.0f ? 1 : -1, v[1] > 0.0f ? 1 : -1, v[2] > 0.0f ? 1 : -1,
+                                  v[3] > 0.0f ? 1 : -1);
+            }
+        }
+        *(u32x4 *)&Js[sr][sc] = jw;
+        *(u32x4 *)&Ss[sr][sc] = sw;
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < EI_KC / 32; ++kk) {
+            const v4i a = *(const v4i *)&Ss[wn * 32 + (lane & 31)][32 * kk + 16 * (lane >> 5)];
+            const v4i b = *(const v4i *)&Js[wm * 32 + (lane & 31)][32 * kk + 16 * (lane >> 5)];
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int i = row0 + wm * 32 + (lane & 31);
+    long long c[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int r = col0 + wn * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int si = (i < N && r < R) ? (X[(size_t)r * ldx + i] > 0.0f ? 1 : -1) : 0;
+        c[v] = (long long)si * (long long)acc[v];
+    }
+    // Sum over the 32 spins of the block (lanes of one half-wave).
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+#pragma unroll
+        for (int m = 16; m >= 1; m >>= 1) c[v] += __shfl_xor(c[v], m);
+    if ((lane & 31) == 0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) red[wm][wn * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)] = c[v];
+    }
+    __syncthreads();
+    if (tid < EI_T && col0 + tid < R) part[(size_t)tm * R + col0 + tid] = red[0][tid] + red[1][tid];
+}
+
+// Hc_r = -E_r / 2 with E_r = sum_i s_i G_i (even: J is symmetric with a zero
+// diagonal), in int64, and its FP64 value (exact below 2^53).
+__global__ __launch_bounds__(256) void k_energy_final_i64(const long long *part, int tiles_m, int R,
+                                                          long long *Hi, double *H)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    long long s = 0;
+    for (int t = 0; t < tiles_m; ++t) s += part[(size_t)t * R + r];
+    const long long hc = -(s / 2);
+    Hi[r] = hc;
+    H[r] = (double)hc;
+}
+
+// The field term Hf_r = -sum_i h_i s_i: one workgroup per replica, a fixed
+// per-thread order and a fixed tree (deterministic; exact for integer h).
+__global__ __launch_bounds__(256) void k_energy_field(const float *h, const float *X, int ldx, int N, double *Hf)
 {
     __shared__ double red[256];
     const int r = blockIdx.x, t = threadIdx.x;
@@ -487,16 +596,17 @@ __global__ __launch_bounds__(256) void k_energy_field(const float *h, const floa
         if (t < w) red[t] += red[t + w];
         __syncthreads();
     }
-    if (t == 0) H[r] -= red[0];
+    if (t == 0) Hf[r] = -red[0];
 }
 
 size_t energy_partial_elems(int N, int R)
 {
+    static_assert(EN_T == EI_T, "both energy forms use one partial layout");
     return (size_t)((N + EN_T - 1) / EN_T) * (size_t)R;
 }
 
-hipError_t launch_energy(const float *J, int ldj, const float *h, const float *X, int ldx, int N, int R,
-                         double *partial, double *H, hipStream_t s)
+hipError_t launch_energy(const float *J, int ldj, const float *X, int ldx, int N, int R, double *partial,
+                         double *H, hipStream_t s)
 {
     if (N <= 0 || R <= 0) return hipSuccess;
     int tiles_m = (N + EN_T - 1) / EN_T, tiles_n = (R + EN_T - 1) / EN_T;
@@ -505,9 +615,26 @@ hipError_t launch_energy(const float *J, int ldj, const float *h, const float *X
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_energy_final, dim3((R + 255) / 256), dim3(256), 0, s, partial, tiles_m, R, H);
-    e = hipGetLastError();
-    if (e != hipSuccess || !h) return e;
-    hipLaunchKernelGGL(k_energy_field, dim3(R), dim3(256), 0, s, h, X, ldx, N, H);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_i8(const float *J, int ldj, const float *X, int ldx, int N, int R, long long *partial,
+                            long long *Hi, double *H, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    if (ldj % EI_KC || ldx % EI_KC || ldj < N || ldx < N) return hipErrorInvalidValue;
+    int tiles_m = (N + EI_T - 1) / EI_T, tiles_n = (R + EI_T - 1) / EI_T;
+    hipLaunchKernelGGL(k_energy_i8, dim3(tiles_m * tiles_n), dim3(256), 0, s, J, ldj, X, ldx, N, R, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_energy_final_i64, dim3((R + 255) / 256), dim3(256), 0, s, partial, tiles_m, R, Hi, H);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_field(const float *h, const float *X, int ldx, int N, int R, double *Hf, hipStream_t s)
+{
+    if (N <= 0 || R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy_field, dim3(R), dim3(256), 0, s, h, X, ldx, N, Hf);
     return hipGetLastError();
 }
 

@@ -43,6 +43,7 @@ struct Request {
     // Results.
     std::vector<double> H;
     std::vector<int64_t> Hi;
+    std::vector<double> cut;
     std::vector<int8_t> S;
     int best = 0, exact = 0;
     double c0 = 0.0;
@@ -196,10 +197,12 @@ void sb_server::run()
                 r->H.assign(R, 0.0);
                 r->Hi.assign(R, 0);
                 r->S.assign((size_t)R * r->N, 0);
+                r->cut.assign(R, 0.0);
                 sb_result res;
                 std::memset(&res, 0, sizeof(res));
                 res.energy_f64 = r->H.data();
                 res.energy_i64 = r->Hi.data();
+                res.cut = r->cut.data();
                 res.spins = r->S.data();
                 std::lock_guard<std::recursive_mutex> g(ctx->mu);
                 st = engine->read(r->slot, &res);
@@ -326,6 +329,7 @@ extern "C" SB_API int sb_server_wait(sb_server *srv, uint64_t ticket, sb_result 
         if (res->energy_f64) std::memcpy(res->energy_f64, r->H.data(), sizeof(double) * R);
         if (res->energy_i64) std::memcpy(res->energy_i64, r->Hi.data(), sizeof(int64_t) * R);
         if (res->spins) std::memcpy(res->spins, r->S.data(), r->S.size());
+        if (res->cut) std::memcpy(res->cut, r->cut.data(), sizeof(double) * R);
         res->best_replica = r->best;
         res->energy_exact = r->exact;
         res->c0_used = r->c0;

@@ -70,7 +70,7 @@ class Result(C.Structure):
         ("spins", C.POINTER(C.c_int8)), ("x_final", C.POINTER(C.c_float)),
         ("y_final", C.POINTER(C.c_float)), ("best_replica", C.c_int32),
         ("energy_exact", C.c_int32), ("c0_used", C.c_double), ("step_ms", C.c_double),
-        ("total_ms", C.c_double),
+        ("total_ms", C.c_double), ("cut", C.POINTER(C.c_double)),
     ]
 
 
@@ -265,6 +265,7 @@ class Solution:
     c0: float
     step_ms: float
     total_ms: float
+    cut: Optional[np.ndarray] = None  # float64 [R], max-cut value of W = -J
 
 
 def _result_struct(R: int, N: int, spins: bool, state: bool):
@@ -273,21 +274,23 @@ def _result_struct(R: int, N: int, spins: bool, state: bool):
     S = np.zeros((R, N), dtype=np.int8) if spins else None
     X = np.zeros((R, N), dtype=np.float32) if state else None
     Y = np.zeros((R, N), dtype=np.float32) if state else None
+    cut = np.zeros(R, dtype=np.float64)
     res = Result()
     res.energy_f64 = H.ctypes.data_as(C.POINTER(C.c_double))
+    res.cut = cut.ctypes.data_as(C.POINTER(C.c_double))
     res.energy_i64 = Hi.ctypes.data_as(C.POINTER(C.c_int64))
     if S is not None:
         res.spins = S.ctypes.data_as(C.POINTER(C.c_int8))
     if X is not None:
         res.x_final = X.ctypes.data_as(C.POINTER(C.c_float))
         res.y_final = Y.ctypes.data_as(C.POINTER(C.c_float))
-    return res, (H, Hi, S, X, Y)
+    return res, (H, Hi, S, X, Y, cut)
 
 
 def _solution(res: Result, bufs) -> Solution:
-    H, Hi, S, X, Y = bufs
+    H, Hi, S, X, Y, cut = bufs
     return Solution(H, Hi if res.energy_exact else None, S, X, Y, int(res.best_replica),
-                    bool(res.energy_exact), float(res.c0_used), float(res.step_ms), float(res.total_ms))
+                    bool(res.energy_exact), float(res.c0_used), float(res.step_ms), float(res.total_ms), cut)
 
 
 class Problem:

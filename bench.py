@@ -13,9 +13,11 @@ Further legs on the same JSON line (each can be skipped):
   serving         Poisson arrivals at 200 req/s (window --serve-seconds), N
                   log-uniform [100,8000], SK/+-1 halves, density U[0.01,1],
                   R in {32,64,128,256}, steps U[1000,5000]: p50/p99 latency
-  accuracy        field error of the FP32 and split paths against an FP64
-                  trajectory (K2000, 8 replicas) and the final-energy gap
-  cpu_baseline    the CPU oracle on a bounded sample of the headline workload
+  accuracy        field error of the FP32 and split paths against FP64
+                  trajectories (SK N=2000 8 replicas, K2000 16 replicas) and
+                  the final-energy gaps per replica
+  c1, cpu_baseline  ER N=800 p=0.06, R=1, 1000 steps, dt=1.25: the GPU thin
+                  path, and the CPU oracle on one thread and on all threads
 """
 import argparse
 import json
@@ -296,15 +298,16 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
     return leg
 
 
-def leg_accuracy(sb, ctx, S, nthreads):
-    from oracle import oracle as O
-    N, R = 2000, 8
-    J = sb.generate(sb.GEN_KN, N, 1.0, seed=3)
+def accuracy_run(sb, ctx, O, kind, N, R, seed, S, nthreads, snaps_per_run=5):
+    """FP64 oracle trajectory of R replicas against the FP32 and split solves
+    at the same seed: the field error of both GPU products on the FP64 state
+    (rounded to FP32) at a few steps, and per-replica final energies."""
+    J = sb.generate(kind, N, 1.0, seed=seed)
     prob = ctx.problem(J)
     c0 = O.c0_auto(O.BSB, J)
-    x32, y32 = O.init_state("uniform", N, R, seed=3)
+    x32, y32 = O.init_state("uniform", N, R, seed=seed)
     x, y = x32.astype(np.float64), y32.astype(np.float64)
-    snaps = [0, S // 4, S // 2, 3 * S // 4, S - 1]
+    snaps = sorted({0, S // 4, S // 2, 3 * S // 4, S - 1}) if snaps_per_run else []
     err = {"fp32": [], "split": []}
     k = 0
     t0 = time.perf_counter()
@@ -321,21 +324,99 @@ def leg_accuracy(sb, ctx, S, nthreads):
     O.run_f64(O.BSB, J, x, y, k, S, S, 1.0, 1.0, c0, 1.0, 2, nthreads)
     oracle_s = time.perf_counter() - t0
     s64 = np.where(x[:, :N] > 0, 1, -1).astype(np.int8)
-    H64 = O.energy_i64(J, s64)
-    out = {"workload": f"K2000 +-1 seed 3, bSB, {R} replicas, {S} steps, dt=1.0; FP64 oracle on {nthreads} threads",
-           "field_error": err, "fp64_mean_energy": float(H64.mean()), "fp64_best_energy": float(H64.min()),
+    H64 = O.energy_i64(J, s64) if kind == sb.GEN_KN else O.energy_f64(J, s64)
+    H64 = np.asarray(H64, dtype=np.float64)
+    out = {"fp64_energy": H64.tolist(), "fp64_mean_energy": float(H64.mean()), "fp64_best_energy": float(H64.min()),
            "oracle_s": oracle_s}
+    if snaps:
+        out["field_error"] = err
     for prec in ("fp32", "split"):
-        sol = ctx.solve(prob, variant="bsb", precision=prec, steps=S, replicas=R, init="uniform", seed=3, dt=1.0,
+        sol = ctx.solve(prob, variant="bsb", precision=prec, steps=S, replicas=R, init="uniform", seed=seed, dt=1.0,
                         c0=c0)
-        out[f"{prec}_mean_energy"] = float(sol.energy.mean())
-        out[f"{prec}_best_energy"] = float(sol.energy.min())
-        out[f"{prec}_mean_gap_vs_fp64"] = (float(sol.energy.mean()) - out["fp64_mean_energy"]) / abs(out["fp64_mean_energy"])
-        out[f"{prec}_best_gap_vs_fp64"] = (float(sol.energy.min()) - out["fp64_best_energy"]) / abs(out["fp64_best_energy"])
-    log(f"accuracy: split mean gap {out['split_mean_gap_vs_fp64']:.2e}, "
-        f"field rel_max split {max(e['rel_max'] for e in err['split']):.2e}")
+        E = sol.energy
+        out[f"{prec}_energy"] = E.tolist()
+        out[f"{prec}_mean_gap_vs_fp64"] = (float(E.mean()) - out["fp64_mean_energy"]) / abs(out["fp64_mean_energy"])
+        out[f"{prec}_best_gap_vs_fp64"] = (float(E.min()) - out["fp64_best_energy"]) / abs(out["fp64_best_energy"])
+        rel = (E - H64) / np.abs(H64)
+        diff = np.nonzero(E != H64)[0]
+        out[f"{prec}_replicas_differing"] = int(diff.size)
+        out[f"{prec}_per_replica_gap"] = {int(r): float(rel[r]) for r in diff}
     prob.close()
     return out
+
+
+def leg_accuracy(sb, ctx, S, nthreads):
+    """FP64 oracle trajectories against the FP32 and split paths.  SK N=2000
+    (BF16-rounded real J in the split form) with 8 replicas, and K2000 +-1
+    (FP8-exact J) with 16 replicas: field error per checkpoint step, final
+    energies per replica and the mean/best gaps."""
+    from oracle import oracle as O
+    out = {"workload": f"bSB, {S} steps, dt=1.0, c0 auto, keyed uniform init; FP64 oracle trajectory on "
+                       f"{nthreads} threads; sk2000: SK N=2000 seed 2, 8 replicas; k2000: K2000 +-1 seed 3, "
+                       "16 replicas",
+           "sk2000": accuracy_run(sb, ctx, O, sb.GEN_SK, 2000, 8, 2, S, nthreads),
+           "k2000": accuracy_run(sb, ctx, O, sb.GEN_KN, 2000, 16, 3, S, nthreads)}
+    for name in ("sk2000", "k2000"):
+        a = out[name]
+        log(f"accuracy {name}: split mean gap {a['split_mean_gap_vs_fp64']:.2e}, fp32 {a['fp32_mean_gap_vs_fp64']:.2e}, "
+            f"field rel_max split {max(e['rel_max'] for e in a['field_error']['split']):.2e}")
+    return out
+
+
+def leg_c1(sb, ctx, nthreads, cpu_seconds=8.0):
+    """The CPU-runnable configuration: ER N=800 p=0.06 +-1 (J = -W), R=1,
+    1000 bSB steps, dt=1.25, a0=1, c0 auto, keyed uniform init from seed 1.
+    GPU: the thin R = 1 path (FP32), timed.  CPU: the oracle's FP32 loop (the
+    reference's sequential-sum order) on one thread, repeated over seeds for a
+    bounded sample, then on all threads (rows of each field split over the
+    threads).  Both report H and the cut value of seed 1."""
+    from oracle import oracle as O
+    N, S, dt = 800, 1000, 1.25
+    J = sb.generate(sb.GEN_ER, N, 0.06, seed=1)
+    prob = ctx.problem(J)
+    kw = dict(variant="bsb", precision="fp32", steps=S, replicas=1, init="uniform", seed=1, dt=dt)
+    ctx.solve(prob, **kw)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = ctx.solve(prob, **kw)
+    gw = (time.perf_counter() - t0) / reps
+    gpu = {"spin_updates_per_s": N * S / gw, "solve_ms": gw * 1e3, "avg_step_us": g.step_ms * 1e3 / S,
+           "H": int(g.energy_i64[0]), "cut": float(g.cut[0]), "c0": g.c0}
+    prob.close()
+    c0 = O.c0_auto(O.BSB, J)
+    W = -np.triu(J.astype(np.float64), 1)
+
+    def cpu_solve(seed, threads):
+        x, y = O.init_state("uniform", N, 1, seed=seed)
+        O.run_f32(O.BSB, J, x, y, 0, S, S, dt, 1.0, c0, nthreads=threads)
+        return x
+
+    t0 = time.perf_counter()
+    n1 = 0
+    while time.perf_counter() - t0 < cpu_seconds or n1 < 3:
+        x = cpu_solve(1 + n1, 1)
+        if n1 == 0:
+            s = O.spins(x, N)
+            H1 = int(O.energy_i64(J, s)[0])
+            cut1 = float(W[s[0][:, None] != s[0][None, :]].sum())
+        n1 += 1
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(n1):
+        cpu_solve(1 + i, nthreads)
+    tn = time.perf_counter() - t0
+    cpu = {"value": N * S * n1 / t1, "unit": "spin-updates/s", "cores": 1, "kind": "port",
+           "sample": f"C1 ER N={N} p=0.06 +-1, R=1, {S} bSB steps, dt={dt}: {n1} solves (seeds 1..{n1}) of the "
+                     f"oracle's FP32 sequential-fmaf loop on 1 thread, {t1:.1f} s, {cpu_model()} "
+                     f"(nproc={os.cpu_count()})",
+           "H_seed1": H1, "cut_seed1": cut1,
+           "all_cores": {"value": N * S * n1 / tn, "cores": nthreads, "seconds": tn,
+                         "how": "the same solves, rows of each field over the threads (same sums)"}}
+    log(f"C1: GPU {gpu['spin_updates_per_s']:.4g} spin-updates/s ({gpu['avg_step_us']:.2f} us/step), "
+        f"CPU 1 thread {cpu['value']:.4g}, {nthreads} threads {cpu['all_cores']['value']:.4g}; "
+        f"H {gpu['H']} / {H1}, cut {gpu['cut']:.0f} / {cut1:.0f}")
+    return gpu, cpu
 
 
 def main() -> int:
@@ -354,7 +435,7 @@ def main() -> int:
     ap.add_argument("--only-headline", action="store_true", help="skip every extra leg")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=1500)
-    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window (60 for the full config)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-drain-cap", type=float, default=120.0, help="max seconds to drain after arrivals")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -435,17 +516,10 @@ def main() -> int:
     if not args.no_accuracy:
         out["accuracy"] = leg_accuracy(sb, ctx, S, threads)
     if not args.no_cpu:
-        from oracle import oracle as O
-        Rc, Sc = 16, S
-        x, y = O.init_state("uniform", N, Rc, seed=2)
-        c0 = O.c0_auto(O.BSB, J)
-        t0 = time.perf_counter()
-        O.run_f32(O.BSB, J, x, y, 0, Sc, Sc, 1.0, 1.0, c0, 1.0, 2, 1)
-        tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": N * Rc * Sc / tc, "unit": "spin-updates/s", "cores": 1, "kind": "port",
-                               "sample": f"same SK N={N} instance, {Rc} replicas x {Sc} steps, FP32 sequential-fmaf "
-                                         f"oracle, 1 thread, {tc:.1f} s on {cpu_model()} (nproc={os.cpu_count()})"}
-        log(f"cpu baseline: {out['cpu_baseline']['value']:.4g} spin-updates/s")
+        gpu_c1, cpu = leg_c1(sb, ctx, threads)
+        out["c1"] = {"workload": "ER N=800 p=0.06 +-1 (J = -W), R=1, 1000 bSB steps, dt=1.25, a0=1, c0 auto, "
+                                 "keyed uniform init seed 1; GPU thin R=1 FP32 path", "gpu": gpu_c1}
+        out["cpu_baseline"] = cpu
     prob.close()
     ctx.close()
     print(json.dumps(out))

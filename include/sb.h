@@ -117,6 +117,9 @@ typedef struct sb_params {
 typedef struct sb_result {
     int64_t *energy_i64;  /* [R] exact H when J is integer-valued             */
     double *energy_f64;   /* [R] H in FP64 (exact for integer-valued J)       */
+    /* Integer-valued J with |J_ij| <= 127: the coupling sum is an exact integer
+     * reduction on the device (int8 J x int8 spins on the i8 MFMA, int32
+     * accumulators, int64 sums); other J: FP64. */
     int8_t *spins;        /* [R*N] s = x > 0 ? +1 : -1, replica-major          */
     float *x_final;       /* [R*N] final x                                    */
     float *y_final;       /* [R*N] final y                                    */
@@ -125,6 +128,10 @@ typedef struct sb_result {
     double c0_used;
     double step_ms;       /* device time of the step loop (HIP events)       */
     double total_ms;      /* host wall time of the whole call                */
+    /* [R] cut value of the max-cut mapping W = -J: the sum of W_ij over the
+     * pairs i < j with s_i != s_j, i.e. (sum_{i<j} W_ij - sum_{i<j} W_ij s_i s_j) / 2
+     * (the field h does not enter).  Exact for integer-valued J.  May be NULL. */
+    double *cut;
 } sb_result;
 
 typedef struct sb_ctx sb_ctx;

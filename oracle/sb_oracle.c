@@ -144,11 +144,16 @@ static inline float sgnf(float v) { return v > 0.0f ? 1.0f : -1.0f; }
 
 /* Rows are processed eight at a time so the sequential fmaf chains of
  * different rows overlap in the CPU pipeline; each row's own order is the
- * plain j = 0..N-1 chain of Bifurcation_m2.cu.c:93-97. */
-static void field_rows_f32(int N, const float *J, int ldj, const float *xs, float *F)
+ * plain j = 0..N-1 chain of Bifurcation_m2.cu.c:93-97.  row_threads > 1
+ * splits the row blocks over OpenMP threads (same sums). */
+static void field_rows_f32(int N, const float *J, int ldj, const float *xs, float *F, int row_threads)
 {
-    int i = 0;
-    for (; i + 8 <= N; i += 8) {
+    const int nb = N / 8;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(row_threads) schedule(static) if (row_threads > 1)
+#endif
+    for (int b = 0; b < nb; ++b) {
+        const int i = 8 * b;
         float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f, a6 = 0.f, a7 = 0.f;
         const float *r0 = J + (size_t)i * ldj;
         for (int j = 0; j < N; ++j) {
@@ -165,7 +170,7 @@ static void field_rows_f32(int N, const float *J, int ldj, const float *xs, floa
         F[i] = a0; F[i + 1] = a1; F[i + 2] = a2; F[i + 3] = a3;
         F[i + 4] = a4; F[i + 5] = a5; F[i + 6] = a6; F[i + 7] = a7;
     }
-    for (; i < N; ++i) {
+    for (int i = 8 * nb; i < N; ++i) {
         float a = 0.f;
         const float *row = J + (size_t)i * ldj;
         for (int j = 0; j < N; ++j) a = fmaf(row[j], xs[j], a);
@@ -204,7 +209,7 @@ static void field_rows_f32_lanes(int N, const float *J, int ldj, const float *xs
 }
 
 static void field_order(int order, int variant, int N, const float *J, int ldj, const float *x,
-                        float *s, float *F);
+                        float *s, float *F, int row_threads);
 
 void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, int ld,
                          int use_sign, float *F)
@@ -212,7 +217,7 @@ void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, 
     float *s = (float *)malloc(sizeof(float) * (size_t)(N + 16));
     for (int r = 0; r < R; ++r)
         field_order(ORC_ORDER_LANES, use_sign ? ORC_DSB : ORC_BSB, N, J, ldj, x + (size_t)r * ld, s,
-                    F + (size_t)r * ld);
+                    F + (size_t)r * ld, 1);
     free(s);
 }
 
@@ -227,26 +232,55 @@ void orc_field_f32(int N, int R, const float *J, int ldj, const float *x, int ld
             for (int j = 0; j < N; ++j) s[j] = sgnf(xr[j]);
             src = s;
         }
-        field_rows_f32(N, J, ldj, src, F + (size_t)r * ld);
+        field_rows_f32(N, J, ldj, src, F + (size_t)r * ld, 1);
     }
     free(s);
+}
+
+/* FP64 field, each row the sequential j = 0..N-1 fma chain; eight rows at a
+ * time (independent chains), row blocks over row_threads OpenMP threads. */
+static void field_rows_f64(int N, const float *J, int ldj, const double *xs, double *F, int row_threads)
+{
+    const int nb = N / 8;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(row_threads) schedule(static) if (row_threads > 1)
+#endif
+    for (int b = 0; b < nb; ++b) {
+        const int i = 8 * b;
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const float *r0 = J + (size_t)i * ldj;
+        for (int j = 0; j < N; ++j) {
+            const double v = xs[j];
+            for (int q = 0; q < 8; ++q) a[q] = fma((double)r0[(size_t)q * ldj + j], v, a[q]);
+        }
+        for (int q = 0; q < 8; ++q) F[i + q] = a[q];
+    }
+    for (int i = 8 * nb; i < N; ++i) {
+        const float *row = J + (size_t)i * ldj;
+        double a = 0.0;
+        for (int j = 0; j < N; ++j) a = fma((double)row[j], xs[j], a);
+        F[i] = a;
+    }
+}
+
+static void field_f64_one(int N, const float *J, int ldj, const double *xr, int use_sign, double *s,
+                          double *F, int row_threads)
+{
+    const double *src = xr;
+    if (use_sign) {
+        for (int j = 0; j < N; ++j) s[j] = xr[j] > 0.0 ? 1.0 : -1.0;
+        src = s;
+    }
+    field_rows_f64(N, J, ldj, src, F, row_threads);
 }
 
 void orc_field_f64(int N, int R, const float *J, int ldj, const double *x, int ld,
                    int use_sign, double *F)
 {
-    for (int r = 0; r < R; ++r) {
-        const double *xr = x + (size_t)r * ld;
-        for (int i = 0; i < N; ++i) {
-            const float *row = J + (size_t)i * ldj;
-            double a = 0.0;
-            for (int j = 0; j < N; ++j) {
-                double v = use_sign ? (xr[j] > 0.0 ? 1.0 : -1.0) : xr[j];
-                a = fma((double)row[j], v, a);
-            }
-            F[(size_t)r * ld + i] = a;
-        }
-    }
+    double *s = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+    for (int r = 0; r < R; ++r)
+        field_f64_one(N, J, ldj, x + (size_t)r * ld, use_sign, s, F + (size_t)r * ld, 1);
+    free(s);
 }
 
 /* ---- updates ----------------------------------------------------------- */
@@ -359,9 +393,10 @@ float orc_m1_y_f32(float F, float x, float y, float p, double a0, double K, doub
 
 /* ---- solver loops ------------------------------------------------------ */
 
-/* Field of one replica in the chosen order; s has round_up(N, 16) entries. */
+/* Field of one replica in the chosen order; s has round_up(N, 16) entries.
+ * row_threads > 1 splits the rows of the SEQ order over OpenMP threads. */
 static void field_order(int order, int variant, int N, const float *J, int ldj, const float *x,
-                        float *s, float *F)
+                        float *s, float *F, int row_threads)
 {
     const int Kp = (N + 15) / 16 * 16;
     const int sign = variant == ORC_DSB;
@@ -371,7 +406,7 @@ static void field_order(int order, int variant, int N, const float *J, int ldj, 
             for (int j = 0; j < N; ++j) s[j] = sgnf(x[j]);
             src = s;
         }
-        field_rows_f32(N, J, ldj, src, F);
+        field_rows_f32(N, J, ldj, src, F, row_threads);
         return;
     }
     for (int j = 0; j < Kp; ++j) {
@@ -390,7 +425,7 @@ static void add_field(int N, const float *h, float *F)
 
 static int run_f32_replica(int order, int variant, int N, const float *J, int ldj, const float *h,
                            float *x, float *y, float *F, float *s, int k0, int k1, int steps,
-                           double dt, double a0, double c0, double K, int M)
+                           double dt, double a0, double c0, double K, int M, int row_threads)
 {
     double dts = dt / (double)M;
     for (int k = k0; k < k1; ++k) {
@@ -401,17 +436,17 @@ static int run_f32_replica(int order, int variant, int N, const float *J, int ld
              * UpdateY with the coupling of the new x. */
             const float p = (float)k / (float)steps;
             for (int i = 0; i < N; ++i) x[i] = orc_m1_x_f32(x[i], y[i], a0, dt);
-            field_order(order, ORC_BSB, N, J, ldj, x, s, F);
+            field_order(order, ORC_BSB, N, J, ldj, x, s, F, row_threads);
             add_field(N, h, F);
             for (int i = 0; i < N; ++i) y[i] = orc_m1_y_f32(F[i], x[i], y[i], p, a0, K, dt, c0);
         } else if (variant == ORC_ASB) {
             /* m2_opt.cu:168-169: local sub-steps, then the kick with new x */
             for (int i = 0; i < N; ++i) orc_asb_local_f32(&x[i], &y[i], cf[3], a0, K, dts, M);
-            field_order(order, variant, N, J, ldj, x, s, F);
+            field_order(order, variant, N, J, ldj, x, s, F, row_threads);
             add_field(N, h, F);
             for (int i = 0; i < N; ++i) y[i] = fmaf(cf[1], F[i], y[i]);
         } else {
-            field_order(order, variant, N, J, ldj, x, s, F);
+            field_order(order, variant, N, J, ldj, x, s, F, row_threads);
             add_field(N, h, F);
             for (int i = 0; i < N; ++i) orc_bsb_update_f32(F[i], cf[0], cf[1], cf[2], &x[i], &y[i]);
         }
@@ -445,8 +480,20 @@ int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int 
     if (rc) return rc;
     if (order != ORC_ORDER_SEQ && order != ORC_ORDER_LANES) return ORC_E_INVALID;
     if (!orc_has_fma()) return ORC_E_NOFMA;
-#ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
+    if (nthreads > 1 && R < nthreads && order == ORC_ORDER_SEQ) {
+        /* Fewer replicas than threads (e.g. R = 1): replicas one after the
+         * other, the rows of each field over the threads (same sums). */
+        float *F = (float *)malloc(sizeof(float) * (size_t)N);
+        float *s = (float *)malloc(sizeof(float) * (size_t)(N + 16));
+        for (int r = 0; r < R; ++r)
+            run_f32_replica(order, variant, N, J, ldj, h, x + (size_t)r * ld, y + (size_t)r * ld,
+                            F, s, k0, k1, steps, dt, a0, c0, K, M, nthreads);
+        free(F);
+        free(s);
+        return ORC_OK;
+    }
+#ifdef _OPENMP
 #pragma omp parallel num_threads(nthreads)
 #endif
     {
@@ -457,7 +504,7 @@ int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int 
 #endif
         for (int r = 0; r < R; ++r)
             run_f32_replica(order, variant, N, J, ldj, h, x + (size_t)r * ld, y + (size_t)r * ld,
-                            F, s, k0, k1, steps, dt, a0, c0, K, M);
+                            F, s, k0, k1, steps, dt, a0, c0, K, M, 1);
         free(F);
         free(s);
     }
@@ -479,12 +526,18 @@ int orc_run_f64_h(int variant, int N, int R, const float *J, int ldj, const floa
     int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
     if (rc) return rc;
     double dts = dt / (double)M;
-#ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
-#pragma omp parallel num_threads(nthreads)
+    /* Fewer replicas than threads: replica-parallel over R threads, the rows
+     * of each field over the remaining share (same sums either way). */
+    const int rep_threads = R < nthreads ? (R > 0 ? R : 1) : nthreads;
+    const int row_threads = nthreads / rep_threads > 1 ? nthreads / rep_threads : 1;
+#ifdef _OPENMP
+    omp_set_max_active_levels(2);
+#pragma omp parallel num_threads(rep_threads)
 #endif
     {
         double *F = (double *)malloc(sizeof(double) * (size_t)N);
+        double *sg = (double *)malloc(sizeof(double) * (size_t)N);
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
@@ -495,7 +548,7 @@ int orc_run_f64_h(int variant, int N, int R, const float *J, int ldj, const floa
                 if (variant == ORC_ASB_M1) {
                     /* the same step in FP64 throughout */
                     for (int i = 0; i < N; ++i) xr[i] = xr[i] + a0 * yr[i] * dt;
-                    orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
+                    field_f64_one(N, J, ldj, xr, 0, sg, F, row_threads);
                     if (h)
                         for (int i = 0; i < N; ++i) F[i] += (double)h[i];
                     for (int i = 0; i < N; ++i) {
@@ -504,12 +557,12 @@ int orc_run_f64_h(int variant, int N, int R, const float *J, int ldj, const floa
                     }
                 } else if (variant == ORC_ASB) {
                     for (int i = 0; i < N; ++i) asb_local_f64(&xr[i], &yr[i], p, a0, K, dts, M);
-                    orc_field_f64(N, 1, J, ldj, xr, ld, 0, F);
+                    field_f64_one(N, J, ldj, xr, 0, sg, F, row_threads);
                     if (h)
                         for (int i = 0; i < N; ++i) F[i] += (double)h[i];
                     for (int i = 0; i < N; ++i) yr[i] = fma(dt * c0, F[i], yr[i]);
                 } else {
-                    orc_field_f64(N, 1, J, ldj, xr, ld, variant == ORC_DSB, F);
+                    field_f64_one(N, J, ldj, xr, variant == ORC_DSB, sg, F, row_threads);
                     if (h)
                         for (int i = 0; i < N; ++i) F[i] += (double)h[i];
                     double alpha = dt * (a0 - a0 * p), beta = dt * c0, gamma = dt * a0;
@@ -518,6 +571,7 @@ int orc_run_f64_h(int variant, int N, int R, const float *J, int ldj, const floa
             }
         }
         free(F);
+        free(sg);
     }
     return ORC_OK;
 }

@@ -173,7 +173,13 @@ def test_sb_field_paths(orc):
         exact = (np.where(xs > 0, 1, -1).astype(np.int64) @ J.astype(np.int64).T).astype(np.float32)
         for prec in ("fp32", "split", "ordered"):
             assert np.array_equal(ctx.field(prob, xs, prec, "dsb"), exact), prec
-        # aSB amplitudes beyond the wall are kept (no clamp in the field)
+        # Amplitudes beyond the wall: no variant clamps or zeroes the field.
         big = 3.0 * xs
-        assert np.allclose(ctx.field(prob, big, "fp32", "asb"), 3.0 * F64, rtol=1e-5, atol=1e-4)
+        for var in ("asb", "bsb", "asb_m1"):
+            assert np.array_equal(ctx.field(prob, big, "ordered", var),
+                                  orc.field_f32(J, np.ascontiguousarray(3.0 * x))[:, :N]), var
+            assert np.allclose(ctx.field(prob, big, "fp32", var), 3.0 * F64, rtol=1e-5, atol=1e-4), var
+            assert np.allclose(ctx.field(prob, big, "split", var), 3.0 * F64, rtol=1e-4, atol=1e-3), var
+        for prec in ("fp32", "split", "ordered"):
+            assert np.array_equal(ctx.field(prob, big, prec, "dsb"), exact), prec
         prob.close()

@@ -116,8 +116,11 @@ def test_fp32_trajectory_statistical_parity(ctx, orc):
     # exact energies for the GPU's own spins
     assert np.array_equal(sol.energy_i64, orc.energy_i64(J, sol.spins))
     assert np.array_equal(sol.energy_i64[same], Ho[same])
-    assert same.mean() >= 0.5, same.mean()
-    assert abs(sol.energy.mean() - Ho.mean()) <= 0.01 * abs(Ho.mean())
+    print(f"N={N} R={R} {steps} steps: identical final spins in {same.sum()}/{R} replicas, "
+          f"mean energy gap {abs(sol.energy.mean() - Ho.mean()) / abs(Ho.mean()):.2e}")
+    # Measured on MI355X: 64 of 64 replicas end with identical spins.
+    assert same.sum() >= 63, same.sum()
+    assert abs(sol.energy.mean() - Ho.mean()) <= 1e-3 * abs(Ho.mean())
 
 
 def test_dsb_integer_all_paths_identical(ctx, orc):
@@ -255,3 +258,36 @@ def test_cli_writes_output_txt(ctx, orc, mode):
     so = orc.spins(np.ascontiguousarray(xo), 40)
     assert np.array_equal(S, so)
     assert np.array_equal(H, orc.energy_i64(J, so))
+
+
+def test_cli_fractional_field_writes_fp64_energies(ctx, orc):
+    # Integer J with a fractional field line: energies are not integers, so
+    # output.txt carries the FP64 values (not truncated integers).
+    N = 30
+    J = kn_pm1(N, 78)
+    h = np.zeros(N, np.float32)
+    h[3], h[17] = 0.5, -1.25
+    lines = [f"{N}"] + [f"{i} {j} {int(J[i, j])}" for i in range(N) for j in range(i + 1, N)]
+    lines += ["3 3 0.5", "17 17 -1.25"]
+    exe = os.path.join(ROOT, "annealing_amd", "bin", "sb_solve")
+    with tempfile.TemporaryDirectory() as d:
+        inst = os.path.join(d, "k30h.txt")
+        open(inst, "w").write("\n".join(lines) + "\n")
+        r = subprocess.run([exe, inst, "--variant", "bsb", "--precision", "ordered", "--steps", "100",
+                            "--replicas", "4", "--init", "uniform", "--seed", "2", "--dt", "0.5",
+                            "--spins", "spins.txt", "--cut", "cut.txt"], cwd=d, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        H = np.loadtxt(os.path.join(d, "output.txt"), dtype=np.float64)
+        S = np.loadtxt(os.path.join(d, "spins.txt"), dtype=np.int8)
+        cut = np.loadtxt(os.path.join(d, "cut.txt"), dtype=np.int64)
+    x, y = orc.init_state("uniform", N, 4, seed=2)
+    c0 = orc.c0_auto(orc.BSB, J)
+    orc.run_f32(orc.BSB, J, x, y, 0, 100, 100, 0.5, 1.0, c0, h=h)
+    so = orc.spins(x, N)
+    assert np.array_equal(S, so)
+    Ho = orc.energy_f64(J, so, h=h)
+    assert np.array_equal(H, Ho)
+    assert np.any(H != np.round(H))
+    W = -np.triu(J.astype(np.int64), 1)
+    assert np.array_equal(cut, [W[so[r][:, None] != so[r][None, :]].sum() for r in range(4)])

@@ -293,6 +293,9 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
         delete c;
         return sbh::hip_fail(e, "sb_ctx_create");
     }
+    // The device is current: look up its static scratch (sb_energy_noalloc).
+    // A failure leaves it null; that entry point then reports it.
+    if (noalloc_scratch_address(device, &c->scratch) != hipSuccess) c->scratch = nullptr;
     const char *at = std::getenv("SB_AUTOTUNE");
     if (at && at[0] == '0') c->autotune = false;
     const char *vb = std::getenv("SB_VERBOSE");
@@ -1374,5 +1377,30 @@ extern "C" SB_API int sb_energy(sb_ctx *ctx, const float *dJ, int32_t N, int32_t
     int rc = sbh::grow(&ctx->part, &ctx->part_cap, need);
     if (rc) return rc;
     HIP_TRY(launch_energy(dJ, ldj, dX, ldx, N, R, ctx->part, dH, s));
+    return SB_OK;
+}
+
+// Variant of sb_energy that allocates nothing: the partial sums go to the
+// device's static scratch (looked up by sb_ctx_create) instead of the
+// context's workspace, which sb_energy grows with hipMalloc.
+extern "C" SB_API int sb_energy_noalloc(sb_ctx *ctx, const float *dJ, int32_t N, int32_t ldj, const float *dX,
+                                        int32_t R, int32_t ldx, double *dH, void *stream)
+{
+    if (N >= 1 && R >= 1 && energy_partial_elems(N, R) > noalloc_scratch_elems()) {
+        sbh::set_error("sb_energy_noalloc: ceil(N / 64) * R = " + std::to_string(energy_partial_elems(N, R)) +
+                       " partial sums exceed the static scratch of " + std::to_string(noalloc_scratch_elems()));
+        return SB_E_INVALID;
+    }
+    if (!ctx || !dJ || !dX || !dH || N < 1 || R < 1 || ldj < N || ldx < N) return SB_E_INVALID;
+    if (!ctx->scratch) {
+        sbh::set_error("sb_energy_noalloc: no static scratch on this device");
+        return SB_E_HIP;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    // Every partial sum the final pass reads is written earlier in the same
+    // call, so the scratch needs no clearing.
+    HIP_TRY(launch_energy(dJ, ldj, dX, ldx, N, R, ctx->scratch, dH, s));
     return SB_OK;
 }

@@ -21,6 +21,8 @@ struct sb_ctx {
     // and the FP32 family (any variant).  Split bSB/aSB always run whole.
     int halves_min_r_fp8 = 1536, halves_min_r_fp32 = 768;
     std::recursive_mutex mu;
+    // This device's static scratch (sb_energy_noalloc); null if the lookup failed.
+    double *scratch = nullptr;
     // Solve workspace (grown on demand, reused across calls).
     float *X0 = nullptr, *X1 = nullptr, *Y = nullptr;
     size_t state_cap = 0;  // floats per buffer

@@ -141,6 +141,10 @@ hipError_t launch_energy_i8(const float *J, int ldj, const float *X, int ldx, in
 // Field part Hf_r = -sum_i h_i s_i.
 hipError_t launch_energy_field(const float *h, const float *X, int ldx, int N, int R, double *Hf, hipStream_t s);
 size_t energy_partial_elems(int N, int R);
+// The static device scratch of sb_energy_noalloc: its capacity in doubles, and
+// its address on `device` (which must be current; looked up once per device).
+size_t noalloc_scratch_elems();
+hipError_t noalloc_scratch_address(int device, double **ptr);
 hipError_t launch_spins(const float *X, int ldx, int N, int R, int8_t *S, hipStream_t s);
 hipError_t launch_compact(const float *X, int ldx, int N, int R, float *out, hipStream_t s);
 hipError_t launch_pack_bf16(const float *J, size_t n_elems, uint16_t *out, hipStream_t s);

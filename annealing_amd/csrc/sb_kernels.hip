@@ -6,6 +6,9 @@
 // device building blocks of sb_tiles.h.
 #include "sb_tiles.h"
 
+#include <map>
+#include <mutex>
+
 namespace sbk {
 
 // Ordered-sum debug kernel: one thread per (spin, replica), F_i summed over
@@ -636,6 +639,38 @@ hipError_t launch_energy_field(const float *h, const float *X, int ldx, int N, i
     if (N <= 0 || R <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_energy_field, dim3(R), dim3(256), 0, s, h, X, ldx, N, Hf);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Static scratch of sb_energy_noalloc: the per-row-tile FP64 partial sums of
+// one call (energy_partial_elems(N, R) doubles), in a file-scope device array
+// instead of allocated memory.  Each GPU the library runs on holds its own
+// copy, at its own address, until the process ends.  In host code the name
+// `scratch` is a host-side stand-in, not a device address: host code passes
+// the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchDoubles = size_t(1) << 20;  // 8 MiB per device
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+size_t noalloc_scratch_elems() { return kScratchDoubles; }
+
+hipError_t noalloc_scratch_address(int device, double **ptr)
+{
+    // Looked up once per device, with that device current (the caller has
+    // made it current), and kept.
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *ptr = it->second;
+        return hipSuccess;
+    }
+    double *p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+    *ptr = p;
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

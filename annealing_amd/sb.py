@@ -38,7 +38,7 @@ EXPORTS = (
     "sb_problem_destroy", "sb_problem_info", "sb_problem_set_field", "sb_problem_set_sparse",
     "sb_problem_sparse_nnz",
     "sb_solve", "sb_step_coeffs_for", "sb_step", "sb_field", "sb_init_state", "sb_asb_prologue",
-    "sb_energy", "sb_batch_solve", "sb_server_create", "sb_server_submit",
+    "sb_energy", "sb_energy_noalloc", "sb_batch_solve", "sb_server_create", "sb_server_submit",
     "sb_server_submit_generated", "sb_server_wait", "sb_server_cancel", "sb_server_stats_get",
     "sb_server_destroy",
 )
@@ -140,6 +140,7 @@ def _bind(lib: C.CDLL) -> None:
         "sb_init_state": (C.c_int, [P, I32, U64, I32, I32, I32, P, P, P]),
         "sb_asb_prologue": (C.c_int, [P, C.POINTER(Params), I32, I32, I32, P, P, P]),
         "sb_energy": (C.c_int, [P, P, I32, I32, P, I32, I32, P, P]),
+        "sb_energy_noalloc": (C.c_int, [P, P, I32, I32, P, I32, I32, P, P]),
         "sb_batch_solve": (C.c_int, [P, I32, P, C.POINTER(Params), C.POINTER(Result), C.POINTER(BatchStats)]),
         "sb_server_create": (C.c_int, [P, C.POINTER(ServerOpts), C.POINTER(P)]),
         "sb_server_submit": (C.c_int, [P, I32, P, I32, C.POINTER(Params), C.POINTER(U64)]),

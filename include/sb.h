@@ -258,6 +258,17 @@ SB_API int sb_asb_prologue(sb_ctx *ctx, const sb_params *p, int32_t N, int32_t R
 SB_API int sb_energy(sb_ctx *ctx, const float *dJ, int32_t N, int32_t ldj,
                      const float *dX, int32_t R, int32_t ldx, double *dH, void *stream);
 
+/* sb_energy that allocates nothing, for callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  No counterpart in the reference.  The partial sums go to a
+ * static device array of 2^20 doubles (8 MiB) that each GPU holds for the
+ * life of the process; calls with ceil(N / 64) * R above 2^20 are rejected
+ * (SB_E_INVALID).  Every call on one GPU shares that array: calls that may
+ * overlap on the device (different streams, or contexts on the same GPU)
+ * race and must be ordered by the caller. */
+SB_API int sb_energy_noalloc(sb_ctx *ctx, const float *dJ, int32_t N, int32_t ldj, const float *dX,
+                             int32_t R, int32_t ldx, double *dH, void *stream);
+
 /* ---- batched solve of many instances (one launch per step) ------------- */
 typedef struct sb_batch_stats {
     int32_t work_items;   /* tiles per step across all instances */

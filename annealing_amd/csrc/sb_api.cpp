@@ -298,6 +298,8 @@ extern "C" SB_API int sb_ctx_create(int32_t device, sb_ctx **out)
     if (noalloc_scratch_address(device, &c->scratch) != hipSuccess) c->scratch = nullptr;
     const char *at = std::getenv("SB_AUTOTUNE");
     if (at && at[0] == '0') c->autotune = false;
+    // Debug override SB_SOLO_GRAPH=0: launch every step from the host.
+    if (const char *sg = std::getenv("SB_SOLO_GRAPH")) c->solo_graphs = sg[0] != '0';
     const char *vb = std::getenv("SB_VERBOSE");
     if (vb && vb[0] != '0') c->verbose = true;
     // Debug override SB_HALVES_MIN_R=<R> for both families (a large value turns halves off).
@@ -345,6 +347,8 @@ extern "C" SB_API int sb_ctx_destroy(sb_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->stream2);
+    for (auto &e : ctx->solo_graph_cache) (void)hipGraphExecDestroy(e.second);
+    ctx->solo_graph_cache.clear();
     free_ws(ctx);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -548,62 +552,73 @@ extern "C" SB_API int sb_problem_set_field(sb_problem *p, const float *h)
     return SB_OK;
 }
 
-extern "C" SB_API int sb_problem_set_sparse(sb_problem *p, int32_t enable)
+namespace sbh {
+
+void free_csr(sb_problem *p)
 {
-    if (!p) return SB_E_INVALID;
-    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
-    HIP_TRY(hipSetDevice(p->ctx->device));
-    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
     (void)hipFree(p->csr_ptr);
     (void)hipFree(p->csr_col);
     (void)hipFree(p->csr_w);
     p->csr_ptr = p->csr_col = nullptr;
     p->csr_w = nullptr;
     p->nnz = 0;
-    if (!enable) return SB_OK;
-    // CSR of the FP32 J, rows in ascending column order.
+}
+
+// CSR form of the problem's FP32 J, built on the device (rows in ascending
+// column order).  Counts the nonzeros first; when they exceed
+// max_density * N^2 nothing is built and p->nnz stays 0.
+int build_csr(sb_problem *p, double max_density)
+{
+    sb_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
     const int N = p->N;
-    std::vector<float> J((size_t)N * N);
-    HIP_TRY(hipMemcpy2D(J.data(), sizeof(float) * (size_t)N, p->J32, sizeof(float) * (size_t)p->ld,
-                        sizeof(float) * (size_t)N, (size_t)N, hipMemcpyDeviceToHost));
+    free_csr(p);
+    int *cnt = nullptr;
+    HIP_TRY(hipMalloc((void **)&cnt, sizeof(int) * (size_t)N));
+    std::vector<int> c(N);
+    hipError_t e = launch_row_nnz(p->J32, N, p->ld, cnt, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c.data(), cnt, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(cnt);
+    if (e != hipSuccess) return hip_fail(e, "sparse J: row counts");
     std::vector<int> ptr((size_t)N + 1, 0);
     long long nnz = 0;
     for (int i = 0; i < N; ++i) {
-        for (int j = 0; j < N; ++j) nnz += J[(size_t)i * N + j] != 0.0f;
+        nnz += c[i];
         if (nnz > INT32_MAX) {
-            sbh::set_error("sparse J: more than 2^31 - 1 nonzeros");
+            set_error("sparse J: more than 2^31 - 1 nonzeros");
             return SB_E_INVALID;
         }
         ptr[(size_t)i + 1] = (int)nnz;
     }
-    std::vector<int> col((size_t)std::max(nnz, 1ll));
-    std::vector<float> w((size_t)std::max(nnz, 1ll));
-    size_t k = 0;
-    for (int i = 0; i < N; ++i)
-        for (int j = 0; j < N; ++j) {
-            const float v = J[(size_t)i * N + j];
-            if (v != 0.0f) {
-                col[k] = j;
-                w[k] = v;
-                ++k;
-            }
-        }
-    hipError_t e = hipMalloc(&p->csr_ptr, sizeof(int) * ptr.size());
-    if (e == hipSuccess) e = hipMalloc(&p->csr_col, sizeof(int) * col.size());
-    if (e == hipSuccess) e = hipMalloc(&p->csr_w, sizeof(float) * w.size());
-    if (e == hipSuccess) e = hipMemcpy(p->csr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->csr_col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->csr_w, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice);
+    if ((double)nnz > max_density * (double)N * (double)N) return SB_OK;
+    const size_t m = (size_t)std::max(nnz, 1ll);
+    e = hipMalloc((void **)&p->csr_ptr, sizeof(int) * ptr.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&p->csr_col, sizeof(int) * m);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->csr_w, sizeof(float) * m);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->csr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_csr_fill(p->J32, N, p->ld, p->csr_ptr, p->csr_col, p->csr_w, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-        (void)hipFree(p->csr_ptr);
-        (void)hipFree(p->csr_col);
-        (void)hipFree(p->csr_w);
-        p->csr_ptr = p->csr_col = nullptr;
-        p->csr_w = nullptr;
-        return sbh::hip_fail(e, "sparse J upload");
+        free_csr(p);
+        return hip_fail(e, "sparse J upload");
     }
     p->nnz = nnz;
     return SB_OK;
+}
+
+}  // namespace sbh
+
+extern "C" SB_API int sb_problem_set_sparse(sb_problem *p, int32_t enable)
+{
+    if (!p) return SB_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream2));
+    sbh::free_csr(p);
+    if (!enable) return SB_OK;
+    return sbh::build_csr(p, 2.0);
 }
 
 extern "C" SB_API int sb_problem_sparse_nnz(const sb_problem *p, int64_t *nnz)
@@ -777,6 +792,13 @@ static int bucket(int v)
     return b;
 }
 
+// A step whose launch costs about as much as its work: the matrix-vector
+// families, or an MFMA step of at most 512 MFLOP (a few microseconds).
+static bool launch_bound(int family, int N, int R)
+{
+    return is_thin(family) || 2.0 * N * (double)N * R <= 512e6;
+}
+
 // Heuristic shape when not tuning: the largest tile that still gives at least
 // one workgroup per CU, else the smallest tile.
 static int heuristic_shape(int family, int N, int R)
@@ -842,12 +864,40 @@ static int tune_candidates(sb_ctx *ctx, const sb_problem *prob, int variant, int
             planes_of = a.bkind;
         }
         HIP_TRY(launch_step(f, s, a, ctx->stream));  // warm-up
-        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        for (int r = 0; r < reps; ++r) HIP_TRY(launch_step(f, s, a, ctx->stream));
-        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-        HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0;
+        int n = reps;
+        if (launch_bound(f, N, R) && ctx->solo_graphs) {
+            // Timed as the solve runs them: back to back in a replayed graph
+            // (host launch cost excluded).
+            n = 16;
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int r = 0; r < n && e == hipSuccess; ++r) e = launch_step(f, s, a, ctx->stream);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hip_fail(e != hipSuccess ? e : e2, "autotune capture");
+            }
+            hipGraphExec_t x = nullptr;
+            e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return hip_fail(e, "autotune instantiate");
+            e = hipGraphLaunch(x, ctx->stream);  // warm-up
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(x, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+            (void)hipGraphExecDestroy(x);
+            if (e != hipSuccess) return hip_fail(e, "autotune graph");
+        } else {
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int r = 0; r < reps; ++r) HIP_TRY(launch_step(f, s, a, ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev1));
+        }
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms /= (float)n / (float)reps;
         if (ctx->verbose) {
             TileShape sh = shape_at(f, s);
             std::fprintf(stderr, "[sb autotune] N=%d R=%d family=%d shape=%d (%dx%d, %d thr) %.2f us\n", N, R, f, s,
@@ -1095,6 +1145,61 @@ static bool two_stream_solve(const sb_ctx *ctx, const sb_params *p, int family, 
     return false;
 }
 
+// The instantiated graph of a whole step loop for `key`, captured on first use
+// (the launches of solve_on_workspace's loop, with the state buffers swapped
+// every step) and kept in the context's cache (at most kSoloGraphs entries,
+// the oldest evicted).
+static int solo_graph(sb_ctx *ctx, const SoloGraphKey &key, StepArgs a, hipGraphExec_t *out)
+{
+    for (auto &e : ctx->solo_graph_cache)
+        if (std::memcmp(&e.first, &key, sizeof(key)) == 0) {
+            *out = e.second;
+            return SB_OK;
+        }
+    hipStream_t s = ctx->stream;
+    float *X = (float *)key.X0, *Xn = (float *)key.X1;
+    unsigned char *P = (unsigned char *)key.P0, *Pn = (unsigned char *)key.P1;
+    sb_params p{};
+    p.variant = key.variant;
+    p.steps = key.steps;
+    p.M = key.M;
+    p.dt = key.dt;
+    p.a0 = key.a0;
+    p.K = key.K;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < key.steps && e == hipSuccess; ++k) {
+        sb_step_coeffs_for(&p, key.c0, k, &a.c);
+        a.X = X;
+        a.Xn = Xn;
+        a.Y = (float *)key.Y;
+        if (a.bkind != BK_NONE) {
+            a.B = P;
+            a.Bn = Pn;
+        }
+        e = launch_step(key.family, key.shape, a, s);
+        std::swap(X, Xn);
+        std::swap(P, Pn);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e != hipSuccess ? e : e2, "solo step graph capture");
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    if (ctx->solo_graph_cache.size() >= kSoloGraphs) {
+        (void)hipGraphExecDestroy(ctx->solo_graph_cache.front().second);
+        ctx->solo_graph_cache.erase(ctx->solo_graph_cache.begin());
+    }
+    ctx->solo_graph_cache.push_back({key, x});
+    *out = x;
+    return SB_OK;
+}
+
 int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double c0, sb_result *res)
 {
     if (prob->nnz > 0 && p->precision != SB_PREC_ORDERED) return solve_sparse(ctx, prob, p, c0, res);
@@ -1137,6 +1242,47 @@ int solve_on_workspace(sb_ctx *ctx, sb_problem *prob, const sb_params *p, double
     if (a.bkind != BK_NONE) {
         HIP_TRY(launch_make_planes(a.bkind, X, R, ld, P, s));
         HIP_TRY(launch_make_planes(a.bkind, X, R, ld, Pn, s));
+    }
+    // Launch-bound solves (a step of a few microseconds) replay the step loop
+    // from a captured graph; the instantiated graph is cached per context,
+    // keyed by every argument of its launches.
+    if (!halves && p->precision != SB_PREC_ORDERED && ctx->solo_graphs && launch_bound(family, N, R)) {
+        SoloGraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.family = family;
+        key.shape = shape;
+        key.J = a.J;
+        key.h = a.h;
+        key.X0 = X;
+        key.X1 = Xn;
+        key.Y = Y;
+        key.P0 = P;
+        key.P1 = Pn;
+        key.bkind = a.bkind;
+        key.N = N;
+        key.R = R;
+        key.ld = ld;
+        key.steps = p->steps;
+        key.variant = p->variant;
+        key.M = p->M;
+        key.dt = p->dt;
+        key.a0 = p->a0;
+        key.c0 = c0;
+        key.K = p->K;
+        hipGraphExec_t exec = nullptr;
+        rc = solo_graph(ctx, key, a, &exec);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ctx->ev0, s));
+        HIP_TRY(hipGraphLaunch(exec, s));
+        HIP_TRY(hipEventRecord(ctx->ev1, s));
+        if (p->steps & 1) std::swap(X, Xn);
+        rc = read_results(ctx, prob, X, Y, R, res, s);
+        if (rc) return rc;
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        res->step_ms = ms;
+        res->c0_used = c0;
+        return SB_OK;
     }
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     if (halves) {

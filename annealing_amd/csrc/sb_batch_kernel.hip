@@ -19,6 +19,8 @@ static const TileShape kBatchFp32[] = {SB_BATCH_FP32(SB_SHAPE_ENTRY)};
 static const TileShape kBatchSplit[] = {SB_BATCH_SPLIT(SB_SHAPE_ENTRY)};
 static const TileShape kBatchFp8[] = {SB_BATCH_FP8(SB_SHAPE_ENTRY)};
 static const TileShape kBatchThin[] = {SB_THIN_ENTRY(1) SB_THIN_ENTRY(2) SB_THIN_ENTRY(4)};
+// Sparse tile: 4 spins (one per wave) x 64 replicas.
+static const TileShape kBatchSparse[] = {TileShape{4, 1, 1, 1, 1}};
 #undef SB_SHAPE_ENTRY
 
 constexpr size_t kBatchLds = 73728;  // max over the shapes above (checked below)
@@ -63,6 +65,7 @@ int batch_shape_count(int family)
     case FAM_THIN_F32:
     case FAM_THIN_BF16:
     case FAM_THIN_FP8: return (int)(sizeof(kBatchThin) / sizeof(kBatchThin[0]));
+    case FAM_SPARSE: return 1;
     default: return 0;
     }
 }
@@ -76,6 +79,7 @@ TileShape batch_shape_at(int family, int idx)
     case FAM_THIN_F32:
     case FAM_THIN_BF16:
     case FAM_THIN_FP8: return kBatchThin[idx];
+    case FAM_SPARSE: return kBatchSparse[idx];
     default: return kBatchFp8[idx];
     }
 }
@@ -158,6 +162,23 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         SB_THIN_CASES(FAM_THIN_BF16, 2)
         SB_THIN_CASES(FAM_THIN_FP8, 1)
 #undef SB_THIN_CASES
+        case (FAM_SPARSE * 2 + 0) * 8 + 0:
+        case (FAM_SPARSE * 2 + 1) * 8 + 0: {
+            SparseArgs sa;
+            sa.ptr = I.csr_ptr;
+            sa.col = I.csr_col;
+            sa.w = I.csr_w;
+            sa.h = I.h;
+            sa.XT = I.XT[k & 1];
+            sa.XTn = I.XT[(k + 1) & 1];
+            sa.YT = I.YT;
+            sa.N = I.N;
+            sa.R = I.R;
+            sa.ldr = I.ldr;
+            sa.c = a.c;
+            sparse_tile(sa, item.tile % I.tiles_m, item.tile / I.tiles_m);
+            break;
+        }
         default: break;
         }
         __syncthreads();  // LDS is reused by the next item

@@ -17,6 +17,9 @@ namespace sbh {
 
 int engine_family(int precision, int variant, const sb_problem *prob, int R)
 {
+    if (precision != SB_PREC_FP32 && precision != SB_PREC_SPLIT) return -1;
+    // A problem with a CSR form runs the sparse family, as sb_solve does.
+    if (prob->nnz > 0) return FAM_SPARSE;
     if (R <= kThinMaxR) {
         // Same family set as a solo solve; FP8 and BF16 thin fields are equal.
         if (precision == SB_PREC_FP32) return FAM_THIN_F32;
@@ -59,20 +62,21 @@ size_t Engine::state_bytes(int N, int R)
     return e * sizeof(float) * 3 + e * plane_bytes(BK_SIGN_BF16) * 2;  // X[2], Y, sign planes[2] (<= 2 B)
 }
 
+size_t Engine::sparse_state_bytes(int N, int R)
+{
+    return (size_t)N * round_up(R, 64) * sizeof(float) * 3;  // XT[2], YT
+}
+
 int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
 {
     const int N = prob->N, ld = prob->ld, R = p.replicas;
-    if (prob->nnz > 0) {
-        set_error("batched solves take dense problems (sb_problem_set_sparse(prob, 0))");
-        return SB_E_INVALID;
-    }
     const int family = engine_family(p.precision, p.variant, prob, R);
     if (family < 0) {
         set_error("batched solve supports SB_PREC_FP32 and SB_PREC_SPLIT");
         return SB_E_INVALID;
     }
     int bshape = 0;
-    int rc = choose_batch_shape(ctx_, prob, family, p.variant, R, &bshape);
+    int rc = family == FAM_SPARSE ? SB_OK : choose_batch_shape(ctx_, prob, family, p.variant, R, &bshape);
     if (rc) return rc;
     int idx = -1;
     for (size_t i = 0; i < slots_.size(); ++i)
@@ -99,6 +103,13 @@ int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
     if (err == hipSuccess) err = hipMalloc((void **)&s.Y, e * sizeof(float));
     if (err == hipSuccess && s.bkind != BK_NONE)
         for (int b = 0; b < 2 && err == hipSuccess; ++b) err = hipMalloc((void **)&s.P[b], e * plane_bytes(s.bkind));
+    if (family == FAM_SPARSE) {
+        s.ldr = round_up(R, 64);
+        s.bytes += sparse_state_bytes(N, R);
+        const size_t et = (size_t)N * s.ldr;
+        for (int b = 0; b < 2 && err == hipSuccess; ++b) err = hipMalloc((void **)&s.XT[b], et * sizeof(float));
+        if (err == hipSuccess) err = hipMalloc((void **)&s.YT, et * sizeof(float));
+    }
     if (err != hipSuccess) {
         s.used = true;
         remove(idx);
@@ -114,6 +125,16 @@ int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
         HIP_TRY(launch_make_planes(s.bkind, s.X[0], R, ld, s.P[0], st));
         HIP_TRY(launch_make_planes(s.bkind, s.X[0], R, ld, s.P[1], st));
     }
+    if (family == FAM_SPARSE) {
+        // Spin-major copy of the initial state, padding replicas zero (as
+        // sb_solve's sparse path).
+        const size_t tb = sizeof(float) * (size_t)N * s.ldr;
+        HIP_TRY(hipMemsetAsync(s.XT[0], 0, tb, st));
+        HIP_TRY(hipMemsetAsync(s.XT[1], 0, tb, st));
+        HIP_TRY(hipMemsetAsync(s.YT, 0, tb, st));
+        HIP_TRY(launch_to_spin_major(s.X[0], ld, N, R, s.XT[0], s.ldr, st));
+        HIP_TRY(launch_to_spin_major(s.Y, ld, N, R, s.YT, s.ldr, st));
+    }
     *slot_out = idx;
     return SB_OK;
 }
@@ -126,8 +147,10 @@ void Engine::remove(int slot)
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(s.X[b]);
         (void)hipFree(s.P[b]);
+        (void)hipFree(s.XT[b]);
     }
     (void)hipFree(s.Y);
+    (void)hipFree(s.YT);
     s = EngineSlot();
 }
 
@@ -179,10 +202,8 @@ int Engine::ensure_tables(size_t n_slots, size_t n_items)
     return SB_OK;
 }
 
-// Work lists: instances are spread over the 8 XCDs (largest first onto the
-// least loaded XCD) so each instance's J and state stay in one XCD's L2; the
-// tiles of an XCD's instances then go, largest first, to that XCD's least
-// loaded workgroup (workgroup w runs on XCD w % 8).
+// Rebuilds the instance table and the per-workgroup item lists (see the
+// placement rules inside).
 int Engine::rebuild()
 {
     if (grid_ <= 0) {
@@ -219,8 +240,20 @@ int Engine::rebuild()
         I.R = R;
         I.family = s.family;
         I.bshape = s.bshape;
-        I.tiles_m = (N + sh.mt() - 1) / sh.mt();
-        I.tiles_n = (R + sh.nt() - 1) / sh.nt();
+        if (s.family == FAM_SPARSE) {
+            I.csr_ptr = s.prob->csr_ptr;
+            I.csr_col = s.prob->csr_col;
+            I.csr_w = s.prob->csr_w;
+            I.XT[0] = s.XT[0];
+            I.XT[1] = s.XT[1];
+            I.YT = s.YT;
+            I.ldr = s.ldr;
+            I.tiles_m = (N + 3) / 4;
+            I.tiles_n = s.ldr / 64;
+        } else {
+            I.tiles_m = (N + sh.mt() - 1) / sh.mt();
+            I.tiles_n = (R + sh.nt() - 1) / sh.nt();
+        }
         I.steps = s.params.steps;
         I.step_base = s.steps_done;
         I.active = s.steps_done < s.params.steps ? 1 : 0;
@@ -233,7 +266,12 @@ int Engine::rebuild()
         if (!I.active) continue;
         // Cost model: MFMA tiles by their area; a thin tile streams its rows of
         // J on the VALU at roughly the cost of a 64-replica MFMA row block.
-        const double tile_cost = (double)sh.mt() * (sh.rows ? 64 : sh.nt()) * (double)round_up(N, kSpinPad);
+        // A sparse tile gathers 4 spins' nonzeros for 64 replicas, at about the
+        // cost of 4 MFMA tile rows per nonzero.
+        const double tile_cost =
+            s.family == FAM_SPARSE
+                ? 4.0 * 64.0 * 4.0 * std::max(1.0, (double)s.prob->nnz / N)
+                : (double)sh.mt() * (sh.rows ? 64 : sh.nt()) * (double)round_up(N, kSpinPad);
         for (int t = 0; t < I.tiles_m * I.tiles_n; ++t) per_slot[i].push_back({tile_cost, (int)i, t});
         slot_cost[i] = tile_cost * I.tiles_m * I.tiles_n;
         total_items += per_slot[i].size();
@@ -241,25 +279,46 @@ int Engine::rebuild()
     rc = ensure_tables(std::max<size_t>(n, 1), std::max<size_t>(total_items, 1));
     if (rc) return rc;
 
+    // Instances with a small J keep all their tiles on one XCD (largest first
+    // onto the least loaded XCD), so J and state stay in that XCD's L2.  An
+    // instance whose J is larger than that is split over the XCDs in bands of
+    // row tiles (XCD x streams band x of J's rows): a few large instances then
+    // still occupy the whole device.  Each XCD's tiles go, largest first, to
+    // its least loaded workgroup (workgroup w runs on XCD w % 8).
     const int nx = 8;
     std::vector<int> order(n);
     for (size_t i = 0; i < n; ++i) order[i] = (int)i;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return slot_cost[a] > slot_cost[b]; });
+    auto spread = [&](int i) {
+        const BatchInstance &I = h_inst_[i];
+        const double bj = I.family == FAM_FP32 || I.family == FAM_THIN_F32 ? 4.0
+                          : I.family == FAM_SPLIT_BF16 || I.family == FAM_THIN_BF16 ? 2.0 : 1.0;
+        const double jbytes = I.family == FAM_SPARSE ? 8.0 * (double)slots_[i].prob->nnz : (double)I.N * I.ldx * bj;
+        return I.tiles_m >= nx && jbytes > 2.0 * (1 << 20);
+    };
     std::vector<double> xcd_load(nx, 0.0);
-    std::vector<std::vector<int>> xcd_slots(nx);
+    std::vector<std::vector<Job>> xcd_jobs(nx);
     for (int i : order) {
-        if (per_slot[i].empty()) continue;
+        if (per_slot[i].empty() || !spread(i)) continue;
+        const int tiles_m = h_inst_[i].tiles_m;
+        for (const Job &j : per_slot[i]) {
+            const int x = (int)((long long)(j.tile % tiles_m) * nx / tiles_m);
+            xcd_jobs[x].push_back(j);
+            xcd_load[x] += j.cost;
+        }
+    }
+    for (int i : order) {
+        if (per_slot[i].empty() || spread(i)) continue;
         int x = (int)(std::min_element(xcd_load.begin(), xcd_load.end()) - xcd_load.begin());
         xcd_load[x] += slot_cost[i];
-        xcd_slots[x].push_back(i);
+        xcd_jobs[x].insert(xcd_jobs[x].end(), per_slot[i].begin(), per_slot[i].end());
     }
     std::vector<std::vector<BatchItem>> wg_items(grid_);
     for (int x = 0; x < nx; ++x) {
         std::vector<int> wgs;
         for (int w = x; w < grid_; w += nx) wgs.push_back(w);
         if (wgs.empty()) wgs.push_back(x % grid_);
-        std::vector<Job> jobs;
-        for (int i : xcd_slots[x]) jobs.insert(jobs.end(), per_slot[i].begin(), per_slot[i].end());
+        std::vector<Job> &jobs = xcd_jobs[x];
         std::stable_sort(jobs.begin(), jobs.end(), [](const Job &a, const Job &b) { return a.cost > b.cost; });
         typedef std::pair<double, int> LW;
         std::priority_queue<LW, std::vector<LW>, std::greater<LW>> heap;
@@ -368,6 +427,10 @@ int Engine::read(int slot, sb_result *res)
     int rc = ensure_workspace(ctx_, 0, R, N);
     if (rc) return rc;
     const int k = s.steps_done;
+    if (s.family == FAM_SPARSE) {
+        HIP_TRY(launch_to_replica_major(s.XT[k & 1], s.ldr, N, R, s.X[k & 1], s.prob->ld, ctx_->stream));
+        HIP_TRY(launch_to_replica_major(s.YT, s.ldr, N, R, s.Y, s.prob->ld, ctx_->stream));
+    }
     rc = read_results(ctx_, s.prob, s.X[k & 1], s.Y, R, res, ctx_->stream);
     if (rc) return rc;
     res->c0_used = s.c0;

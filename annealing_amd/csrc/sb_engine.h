@@ -23,6 +23,11 @@ struct EngineSlot {
     float *X[2] = {nullptr, nullptr};
     float *Y = nullptr;
     unsigned char *P[2] = {nullptr, nullptr};
+    // Sparse family: spin-major state [N][ldr] (X and Y above stay
+    // replica-major for init and readout).
+    float *XT[2] = {nullptr, nullptr};
+    float *YT = nullptr;
+    int ldr = 0;
     size_t bytes = 0;   // device bytes of this slot's state
     int steps_done = 0;
     bool used = false;  // slot holds an instance
@@ -35,8 +40,10 @@ class Engine {
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
 
-    // Device bytes add() would allocate for this instance.
+    // Device bytes add() would allocate for this instance (dense families;
+    // a sparse one adds sparse_state_bytes).
     static size_t state_bytes(int N, int R);
+    static size_t sparse_state_bytes(int N, int R);
     // Choose family/shape, allocate state, run init (+ aSB prologue, planes).
     int add(sb_problem *prob, const sb_params &p, double c0, int *slot);
     // Free the slot's state.

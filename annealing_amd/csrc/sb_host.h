@@ -9,6 +9,15 @@
 
 #include "sb_internal.h"
 
+// Everything the launches of a solo step loop depend on (the key of the
+// context's cache of instantiated step-loop graphs).  Compared bytewise.
+struct SoloGraphKey {
+    int family, shape, bkind, N, R, ld, steps, variant, M, pad;
+    const void *J, *h, *X0, *X1, *Y, *P0, *P1;
+    double dt, a0, c0, K;
+};
+constexpr size_t kSoloGraphs = 8;
+
 struct sb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -43,6 +52,9 @@ struct sb_ctx {
     // Autotuned (family, shape) per (family set, N bucket, R bucket).
     std::map<std::tuple<int, int, int, int>, std::pair<int, int>> tune;
     bool autotune = true;
+    // Launch-bound solves replay their step loop from a cached graph.
+    bool solo_graphs = true;
+    std::vector<std::pair<SoloGraphKey, hipGraphExec_t>> solo_graph_cache;
     bool verbose = false;   // SB_VERBOSE=1: log autotune timings to stderr
     int live_problems = 0;  // guarded by mu
 };
@@ -79,6 +91,10 @@ int check_params(const sb_params *p, int N, double sigma, double *c0_out);
 bool has_prologue(int variant);
 // Sparse family helpers (spin-major workspace, kernel arguments of a problem).
 int ensure_sparse_ws(sb_ctx *ctx, size_t elems);
+// CSR form of a problem's J built on the device; nothing is built (nnz stays
+// 0) when J has more than max_density * N^2 nonzeros.  free_csr drops it.
+int build_csr(sb_problem *p, double max_density);
+void free_csr(sb_problem *p);
 sbk::SparseArgs sparse_args(const sb_problem *prob, int R, int ldr);
 sb_step_coeffs prologue_coeffs(const sb_params &p);
 

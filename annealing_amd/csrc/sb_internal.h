@@ -92,7 +92,8 @@ enum Family {
     FAM_THIN_F32 = 4,   // R <= kThinMaxR: FP32 J streamed, FP32 FMA on the VALU
     FAM_THIN_BF16 = 5,  // same with BF16 J
     FAM_THIN_FP8 = 6,   // same with FP8 J
-    FAM_COUNT = 7
+    FAM_SPARSE = 7,     // CSR J gathered over a spin-major state (batched launches)
+    FAM_COUNT = 8
 };
 inline bool is_thin(int family) { return family >= FAM_THIN_F32 && family <= FAM_THIN_FP8; }
 
@@ -118,6 +119,11 @@ struct SparseArgs {
     sb_step_coeffs c;
 };
 hipError_t launch_step_sparse(const SparseArgs &a, hipStream_t s);
+// CSR of a dense FP32 J [N][ld] on the device: nonzeros per row into cnt [N];
+// then, with ptr [N + 1] their prefix sums, columns and values per row in
+// ascending column order.
+hipError_t launch_row_nnz(const float *J, int N, int ld, int *cnt, hipStream_t s);
+hipError_t launch_csr_fill(const float *J, int N, int ld, const int *ptr, int *col, float *w, hipStream_t s);
 // [R][ldx] replica-major <-> [N][ldr] spin-major (rows/columns past R or N untouched).
 hipError_t launch_to_spin_major(const float *X, int ldx, int N, int R, float *XT, int ldr, hipStream_t s);
 hipError_t launch_to_replica_major(const float *XT, int ldr, int N, int R, float *X, int ldx, hipStream_t s);
@@ -172,6 +178,12 @@ struct BatchInstance {
     int active;          // 0: skip (slot free or finished)
     int variant, M;
     double dt, a0, c0, K;
+    // Sparse family: CSR J and the spin-major state ([N][ldr], ldr = round_up(R, 64)).
+    const int *csr_ptr, *csr_col;
+    const float *csr_w;
+    float *XT[2];
+    float *YT;
+    int ldr;
 };
 
 struct BatchItem {

@@ -118,40 +118,9 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
 }
 
 // ---------------------------------------------------------------------------
-// Sparse family (CSR J, spin-major state).  For low-density J the field is a
-// gather: wave w of workgroup t owns spin i = 4t + w and the 64 replicas
-// 64 blockIdx.y + lane (lane = replica).  For each nonzero (j, w_ij) of row i,
-// in ascending j, the wave reads the 64 consecutive replicas of X^T[j] (one
-// 256-byte load) and accumulates fmaf(w_ij, x_j, acc): the sequential
-// j = 0..N-1 chain of Bifurcation_m2.cu.c:93-97 without its zero terms (a zero
-// term changes at most the sign of a zero sum), so it equals the ordered path.
-// The row's indices and values are wave-uniform (scalar loads).
+// Sparse family: one workgroup per 4 spins x 64 replicas (sparse_tile).
 
-__global__ __launch_bounds__(256) void k_step_sparse(SparseArgs a)
-{
-    const int lane = threadIdx.x & 63;
-    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (i >= a.N) return;
-    const int r = blockIdx.y * 64 + lane;  // < ldr: padding replicas hold 0
-    const bool sgn = a.c.variant == SB_DSB;
-    const int k0 = a.ptr[i], k1 = a.ptr[i + 1];
-    const float *xr = a.XT + r;
-    float acc = 0.0f;
-#pragma unroll 8
-    for (int k = k0; k < k1; ++k) {
-        const int j = a.col[k];
-        float v = xr[(size_t)j * a.ldr];
-        if (sgn) v = sgn1(v);
-        acc = __builtin_fmaf(a.w[k], v, acc);
-    }
-    if (r >= a.R) return;
-    if (a.h) acc = acc + a.h[i];  // external field, after the sum
-    const size_t o = (size_t)i * a.ldr + r;
-    float x = a.XT[o], y = a.YT[o];
-    sb_update(a.c, acc, x, y);
-    a.XTn[o] = x;
-    a.YT[o] = y;
-}
+__global__ __launch_bounds__(256) void k_step_sparse(SparseArgs a) { sparse_tile(a, blockIdx.x, blockIdx.y); }
 
 hipError_t launch_step_sparse(const SparseArgs &a, hipStream_t s)
 {
@@ -638,6 +607,64 @@ hipError_t launch_energy_field(const float *h, const float *X, int ldx, int N, i
 {
     if (N <= 0 || R <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_energy_field, dim3(R), dim3(256), 0, s, h, X, ldx, N, Hf);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// CSR form of a dense J on the device: nonzeros per row, then (after a host
+// prefix sum into ptr) the column indices and values of each row in
+// ascending column order.
+
+__global__ __launch_bounds__(256) void k_row_nnz(const float *J, int N, int ld, int *cnt)
+{
+    __shared__ int red[4];
+    const int i = blockIdx.x, t = threadIdx.x;
+    int c = 0;
+    for (int j = t; j < N; j += 256) c += J[(size_t)i * ld + j] != 0.0f;
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
+    if ((t & 63) == 0) red[t >> 6] = c;
+    __syncthreads();
+    if (t == 0) cnt[i] = red[0] + red[1] + red[2] + red[3];
+}
+
+// One workgroup per row; each pass over 256 columns places its nonzeros by a
+// workgroup prefix count (wave ballots), so the order stays ascending.
+__global__ __launch_bounds__(256) void k_csr_fill(const float *J, int N, int ld, const int *ptr, int *col,
+                                                  float *w)
+{
+    __shared__ int wave_cnt[4];
+    const int i = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int base = ptr[i];
+    for (int j0 = 0; j0 < N; j0 += 256) {
+        const int j = j0 + t;
+        const float v = j < N ? J[(size_t)i * ld + j] : 0.0f;
+        const bool nz = v != 0.0f;
+        const unsigned long long m = __ballot(nz);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_cnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int q = 0; q < wave; ++q) off += wave_cnt[q];
+        if (nz) {
+            col[off + before] = j;
+            w[off + before] = v;
+        }
+        base += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        __syncthreads();
+    }
+}
+
+hipError_t launch_row_nnz(const float *J, int N, int ld, int *cnt, hipStream_t s)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_nnz, dim3(N), dim3(256), 0, s, J, N, ld, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_fill(const float *J, int N, int ld, const int *ptr, int *col, float *w, hipStream_t s)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_csr_fill, dim3(N), dim3(256), 0, s, J, N, ld, ptr, col, w);
     return hipGetLastError();
 }
 

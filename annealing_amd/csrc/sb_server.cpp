@@ -1,15 +1,24 @@
 // sb_server.cpp -- continuous-batching request server.
 //
 // A worker thread owns one Engine (device-resident J and state per request).
+// An instance with at most 2 % nonzeros runs the sparse family (its CSR form
+// is built on the device at admission).
 // Between chunks of `chunk_steps` steps it retires finished requests (energies
-// and spins read back, device memory freed) and admits queued requests in
-// arrival order while their J + state fit the HBM budget (smaller requests may
-// pass a head request that does not fit yet).  All resident requests advance
-// together in one replayed chunk graph; a request's arithmetic depends only on
-// its own (N, R, params), so its result is the same alone and in any mix.
+// and spins read back, device memory freed) and admits queued requests while
+// the resident set stays within two bounds: the HBM budget (J + state of every
+// resident request; a request that does not fit yet lets smaller ones pass)
+// and a tile budget of about two work items per workgroup of the persistent
+// grid, which is enough to keep every CU busy.  Admitting more than that would
+// not raise throughput, only stretch every resident request's steps
+// (processor sharing).  The queue is ordered by total work (N^2 R steps,
+// shortest first; ties in arrival order) or by arrival (SB_SERVE_ARRIVAL).
+// All resident requests advance together in one replayed chunk graph; a
+// request's arithmetic depends only on its own (N, R, params), so its result
+// is the same alone and in any mix.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -35,6 +44,9 @@ struct Request {
     double gen_p = 0.0;
     uint64_t gen_seed = 0;
     size_t bytes = 0;  // device bytes while resident
+    size_t sparse_bytes = 0;  // CSR + spin-major state when admitted as sparse
+    double work = 0.0; // N^2 R steps (queue order of SB_SERVE_SHORTEST)
+    long long tiles = 0;  // work items per step (estimate, for the tile budget)
     sb_problem *prob = nullptr;
     int slot = -1;
     ReqState state = Q_QUEUED;
@@ -49,6 +61,14 @@ struct Request {
     double c0 = 0.0;
     clk::time_point t_submit, t_done;
 };
+
+// Work items per step of a request, from the batched tile shapes: MFMA tiles
+// of about 128 spins x 64 replicas, or thin tiles of 16 spins for R <= 4.
+long long tile_estimate(int N, int R)
+{
+    if (R <= sbk::kThinMaxR) return (N + 15) / 16;
+    return (long long)((N + 127) / 128) * ((R + 63) / 64);
+}
 
 size_t problem_bytes(int N)
 {
@@ -71,6 +91,9 @@ struct sb_server {
     std::deque<uint64_t> queue;
     std::vector<uint64_t> active;
     size_t bytes_in_use = 0;
+    long long active_tiles = 0, tile_budget = 0;
+    // Instances with at most this fraction of nonzeros run the sparse family.
+    double sparse_density = 0.02;
     sb_server_stats stats{};
 
     void run();
@@ -107,6 +130,15 @@ int sb_server::admit_one(Request *r)
     rc = sbh::check_params(&r->params, r->N, r->prob->sigma, &r->c0);
     if (rc) return rc;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    // Low-density instances run the sparse family (a gather per nonzero
+    // instead of the dense product).
+    if (sparse_density > 0.0) {
+        rc = sbh::build_csr(r->prob, sparse_density);
+        if (rc) return rc;
+        if (r->prob->nnz > 0)
+            r->sparse_bytes = (size_t)r->prob->nnz * 8 + ((size_t)r->N + 1) * 4 +
+                              sbh::Engine::sparse_state_bytes(r->N, r->params.replicas);
+    }
     return engine->add(r->prob, r->params, r->c0, &r->slot);
 }
 
@@ -131,11 +163,14 @@ void sb_server::run()
                 }
                 const int cap = opts.max_active > 0 ? opts.max_active : 1 << 30;
                 if ((int)(active.size() + to_admit.size()) >= cap) break;
+                // Tile budget (the first request is always admitted).
+                if (active_tiles > 0 && active_tiles + r->tiles > tile_budget) break;
                 if (bytes_in_use + r->bytes > opts.hbm_budget_bytes) {
                     ++it;
                     continue;
                 }
                 bytes_in_use += r->bytes;
+                active_tiles += r->tiles;
                 to_admit.push_back(r);
                 it = queue.erase(it);
             }
@@ -146,9 +181,13 @@ void sb_server::run()
             std::lock_guard<std::mutex> lk(mu);
             if (rc) {
                 bytes_in_use -= r->bytes;
+                active_tiles -= r->tiles;
                 finish(r, rc);
                 cv_done.notify_all();
             } else {
+                // The sparse form's extra bytes, known only after generation.
+                r->bytes += r->sparse_bytes;
+                bytes_in_use += r->sparse_bytes;
                 r->state = Q_ACTIVE;
                 active.push_back(r->ticket);
                 changed = true;
@@ -163,6 +202,7 @@ void sb_server::run()
                     std::lock_guard<std::recursive_mutex> g(ctx->mu);
                     finish(r, SB_E_CANCELLED);
                     bytes_in_use -= r->bytes;
+                    active_tiles -= r->tiles;
                     ++stats.cancelled;
                     it = active.erase(it);
                     changed = true;
@@ -215,6 +255,7 @@ void sb_server::run()
                 finish(r, st);
             }
             bytes_in_use -= r->bytes;
+            active_tiles -= r->tiles;
             active.erase(std::find(active.begin(), active.end(), r->ticket));
             ++stats.completed;
             cv_done.notify_all();
@@ -230,6 +271,7 @@ extern "C" SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, 
 {
     if (!ctx || !opts || !out || opts->hbm_budget_bytes == 0 || opts->chunk_steps < 1) return SB_E_INVALID;
     if (opts->precision != SB_PREC_FP32 && opts->precision != SB_PREC_SPLIT) return SB_E_INVALID;
+    if (opts->policy != SB_SERVE_SHORTEST && opts->policy != SB_SERVE_ARRIVAL) return SB_E_INVALID;
     sb_server *s = new sb_server();
     s->ctx = ctx;
     s->opts = *opts;
@@ -244,6 +286,9 @@ extern "C" SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, 
             delete s;
             return SB_E_HIP;
         }
+        s->tile_budget = 2LL * s->engine->grid();
+        if (const char *tb = std::getenv("SB_SERVE_TILE_BUDGET")) s->tile_budget = std::max(1LL, std::atoll(tb));
+        if (const char *sd = std::getenv("SB_SERVE_SPARSE_DENSITY")) s->sparse_density = std::atof(sd);
     }
     s->worker = std::thread([s] { s->run(); });
     *out = s;
@@ -261,6 +306,8 @@ static int submit_common(sb_server *srv, std::unique_ptr<Request> r, uint64_t *t
     int rc = sbh::check_params(&p, r->N, 1.0, &c0);
     if (rc) return rc;
     r->bytes = problem_bytes(r->N) + sbh::Engine::state_bytes(r->N, p.replicas);
+    r->work = (double)r->N * r->N * p.replicas * p.steps;
+    r->tiles = tile_estimate(r->N, p.replicas);
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return SB_E_SHUTDOWN;
     ++srv->stats.submitted;
@@ -272,7 +319,15 @@ static int submit_common(sb_server *srv, std::unique_ptr<Request> r, uint64_t *t
     r->ticket = srv->next_ticket++;
     r->t_submit = clk::now();
     *ticket = r->ticket;
-    srv->queue.push_back(r->ticket);
+    if (srv->opts.policy == SB_SERVE_ARRIVAL) {
+        srv->queue.push_back(r->ticket);
+    } else {
+        // Shortest total work first; equal work keeps arrival order.
+        const double w = r->work;
+        auto pos = std::upper_bound(srv->queue.begin(), srv->queue.end(), w,
+                                    [&](double v, uint64_t t) { return v < srv->reqs[t]->work; });
+        srv->queue.insert(pos, r->ticket);
+    }
     srv->reqs[r->ticket] = std::move(r);
     srv->cv_work.notify_all();
     return SB_OK;

@@ -864,6 +864,41 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
 }
 
 // ---------------------------------------------------------------------------
+// Sparse family (CSR J, spin-major state).  For low-density J the field is a
+// gather: wave w of tile (tm, tn) owns spin i = 4 tm + w and the 64 replicas
+// 64 tn + lane (lane = replica).  For each nonzero (j, w_ij) of row i, in
+// ascending j, the wave reads the 64 consecutive replicas of X^T[j] (one
+// 256-byte load) and accumulates fmaf(w_ij, x_j, acc): the sequential
+// j = 0..N-1 chain of Bifurcation_m2.cu.c:93-97 without its zero terms (a zero
+// term changes at most the sign of a zero sum), so it equals the ordered path.
+// The row's indices and values are wave-uniform (scalar loads).  No barriers.
+__device__ __forceinline__ void sparse_tile(const SparseArgs &a, int tm, int tn)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane(tm * 4 + (threadIdx.x >> 6));
+    if (i >= a.N) return;
+    const int r = tn * 64 + lane;  // < ldr: padding replicas hold 0
+    const bool sgn = a.c.variant == SB_DSB;
+    const int k0 = a.ptr[i], k1 = a.ptr[i + 1];
+    const float *xr = a.XT + r;
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int k = k0; k < k1; ++k) {
+        const int j = a.col[k];
+        float v = xr[(size_t)j * a.ldr];
+        if (sgn) v = sgn1(v);
+        acc = __builtin_fmaf(a.w[k], v, acc);
+    }
+    if (r >= a.R) return;
+    if (a.h) acc = acc + a.h[i];  // external field, after the sum
+    const size_t o = (size_t)i * a.ldr + r;
+    float x = a.XT[o], y = a.YT[o];
+    sb_update(a.c, acc, x, y);
+    a.XTn[o] = x;
+    a.YT[o] = y;
+}
+
+// ---------------------------------------------------------------------------
 // Shape tables (4-wave and 8-wave workgroups).
 
 #define SB_FP32_SHAPES(X) X(2, 2, 1, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(1, 4, 1, 1) X(4, 1, 1, 1) \

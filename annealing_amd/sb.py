@@ -28,6 +28,8 @@ GEN_ER, GEN_SK, GEN_KN, GEN_SK_DILUTED = 0, 1, 2, 3
 VARIANTS = {"asb": ASB, "bsb": BSB, "dsb": DSB, "asb_m1": ASB_M1}
 PRECISIONS = {"fp32": PREC_FP32, "split": PREC_SPLIT, "ordered": PREC_ORDERED}
 INITS = {"uniform": INIT_UNIFORM, "reference": INIT_REFERENCE}
+SERVE_SHORTEST, SERVE_ARRIVAL = 0, 1
+SERVE_POLICIES = {"shortest": SERVE_SHORTEST, "arrival": SERVE_ARRIVAL}
 
 # Every symbol include/sb.h declares.
 EXPORTS = (
@@ -93,7 +95,7 @@ class BatchStats(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_size_t), ("chunk_steps", C.c_int32),
-        ("max_active", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("max_active", C.c_int32), ("precision", C.c_int32), ("policy", C.c_int32),
     ]
 
 
@@ -431,7 +433,7 @@ class Server:
     """Continuous-batching request server (sb_server_* of include/sb.h)."""
 
     def __init__(self, ctx: Context, hbm_budget_bytes: int, chunk_steps: int = 50, max_active: int = 0,
-                 precision="split"):
+                 precision="split", policy="shortest"):
         self.ctx = ctx
         self.lib = ctx.lib
         o = ServerOpts()
@@ -439,6 +441,7 @@ class Server:
         o.chunk_steps = chunk_steps
         o.max_active = max_active
         o.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        o.policy = SERVE_POLICIES[policy] if isinstance(policy, str) else policy
         h = C.c_void_p()
         _check(self.lib.sb_server_create(ctx.handle, C.byref(o), C.byref(h)), "sb_server_create")
         self.handle = h

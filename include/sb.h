@@ -289,12 +289,21 @@ SB_API int sb_batch_solve(sb_ctx *ctx, int32_t n, sb_problem *const *probs,
 /* ---- serving (continuous batching) -------------------------------------- */
 typedef struct sb_server sb_server;
 
+/* Queue order of the server.  Resident requests are admitted while their J
+ * and state fit the HBM budget and the resident set stays within a tile
+ * budget of about two work items per workgroup of the persistent grid (the
+ * device is then busy; more resident requests would only share it). */
+typedef enum sb_serve_policy {
+    SB_SERVE_SHORTEST = 0,  /* least total work (N^2 R steps) first; ties by arrival */
+    SB_SERVE_ARRIVAL = 1    /* arrival order (smaller requests backfill the HBM budget) */
+} sb_serve_policy;
+
 typedef struct sb_server_opts {
     size_t hbm_budget_bytes;  /* device bytes for J + state of admitted requests */
     int32_t chunk_steps;      /* steps run between admission points (>= 1)     */
-    int32_t max_active;       /* cap on concurrently resident requests          */
+    int32_t max_active;       /* cap on concurrently resident requests (0: none) */
     int32_t precision;        /* sb_precision for all requests                   */
-    int32_t reserved;
+    int32_t policy;           /* sb_serve_policy                                 */
 } sb_server_opts;
 
 typedef struct sb_server_stats {

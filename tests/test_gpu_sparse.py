@@ -1,6 +1,8 @@
 """Sparse family (CSR J, spin-major state): the field is the sequential fmaf
 chain over the nonzeros in ascending column order, so a sparse solve equals the
 oracle's sequential (SEQ) order bit for bit, like SB_PREC_ORDERED."""
+import os
+
 import numpy as np
 import pytest
 
@@ -74,9 +76,58 @@ def test_sparse_field_operator(ctx, orc):
     prob.close()
 
 
-def test_sparse_problem_rejected_by_batch(ctx):
-    prob = sparse_problem(ctx, sb.generate(sb.GEN_ER, 200, 0.05, 84))
-    params = [sb.make_params(variant="bsb", precision="fp32", steps=10, replicas=16)]
-    with pytest.raises(sb.SBError):
-        ctx.batch_solve([prob], params)
-    prob.close()
+@pytest.mark.parametrize("precision", ["fp32", "split"])
+def test_sparse_problems_in_a_batch(ctx, orc, precision):
+    # Sparse and dense problems batched together: the sparse instances run the
+    # gather tiles inside the persistent launch and equal both their solo
+    # sparse solve and the oracle's sequential order, bit for bit.
+    specs = [("sparse", sb.GEN_ER, 600, 0.02, "bsb", 48, 80), ("dense", sb.GEN_KN, 300, 1.0, "bsb", 64, 60),
+             ("sparse", sb.GEN_SK_DILUTED, 1500, 0.01, "dsb", 130, 50), ("sparse", sb.GEN_ER, 257, 0.05, "asb", 3, 70),
+             ("dense", sb.GEN_SK, 400, 1.0, "dsb", 2, 40)]
+    probs, params, Js = [], [], []
+    for i, (form, kind, N, p, v, R, steps) in enumerate(specs):
+        J = sb.generate(kind, N, p, 90 + i)
+        prob = ctx.problem(J)
+        if form == "sparse":
+            prob.set_sparse(True)
+            assert prob.nnz == int(np.count_nonzero(J))
+        probs.append(prob)
+        Js.append(J)
+        params.append(sb.make_params(variant=v, precision=precision, steps=steps, replicas=R, init="uniform",
+                                     seed=i, dt=1.0 if v == "asb" else 0.5))
+    sols, st = ctx.batch_solve(probs, params, spins=True, state=True)
+    assert st.graph_used == 1
+    for i, (form, kind, N, p, v, R, steps) in enumerate(specs):
+        kw = dict(state=True, variant=v, precision=precision, steps=steps, replicas=R, init="uniform", seed=i,
+                  dt=1.0 if v == "asb" else 0.5)
+        solo = ctx.solve(probs[i], **kw)
+        assert np.array_equal(sols[i].x, solo.x) and np.array_equal(sols[i].y, solo.y), i
+        assert np.array_equal(sols[i].energy, solo.energy), i
+        if form == "sparse":
+            xo, yo, _ = orc.solve_f32(sb.VARIANTS[v], Js[i], R, steps, init="uniform", seed=i, dt=kw["dt"])
+            assert np.array_equal(sols[i].x, xo[:, :N]) and np.array_equal(sols[i].y, yo[:, :N]), i
+    for prob in probs:
+        prob.close()
+
+
+def test_server_runs_low_density_requests_sparse(ctx, orc):
+    # At most 2 % nonzeros: the server builds the CSR form at admission and the
+    # request follows the oracle's sequential order (bSB on real J included).
+    GB = 1 << 30
+    specs = [(sb.GEN_SK_DILUTED, 2000, 0.01, 31, "bsb", 64, 120), (sb.GEN_ER, 700, 0.015, 32, "dsb", 33, 90),
+             (sb.GEN_ER, 500, 0.5, 33, "bsb", 32, 60)]
+    with sb.Server(ctx, 8 * GB, chunk_steps=16, precision="split") as srv:
+        tickets = [srv.submit_generated(k, N, p, g, variant=v, precision="split", steps=s, replicas=R,
+                                        init="uniform", seed=g, dt=0.5) for k, N, p, g, v, R, s in specs]
+        res = [srv.wait(t) for t in tickets]
+    for (k, N, p, g, v, R, s), r in zip(specs, res):
+        prob = ctx.generate(k, N, p, g)
+        J = prob.read_j()
+        if p <= 0.02:
+            prob.set_sparse(True)
+            xo, _, _ = orc.solve_f32(sb.VARIANTS[v], J, R, s, init="uniform", seed=g, dt=0.5,
+                                     nthreads=min(16, os.cpu_count() or 1))
+            assert np.array_equal(r.spins, orc.spins(np.ascontiguousarray(xo), N)), N
+        solo = ctx.solve(prob, variant=v, precision="split", steps=s, replicas=R, init="uniform", seed=g, dt=0.5)
+        assert np.array_equal(r.spins, solo.spins) and np.array_equal(r.energy, solo.energy), N
+        prob.close()

@@ -147,3 +147,35 @@ def test_thin_sb_step_matches_solve(orc):
             assert np.array_equal(X.cpu().numpy()[:, :N], sol.x), jfmt
             assert np.array_equal(Y.cpu().numpy()[:, :N], sol.y), jfmt
         prob.close()
+
+
+def test_solo_graph_replays_equal_eager_launches(orc):
+    # Launch-bound solves (R <= 4, or small MFMA steps) replay their step loop
+    # from a cached graph.  Every replay, with other eager solves and
+    # synchronisations in between, must equal the step-by-step launches.
+    import os
+    cases = [(sb.GEN_ER, 800, 0.06, "bsb", "fp32", 1, 300), (sb.GEN_KN, 2000, 1.0, "dsb", "split", 2, 150),
+             (sb.GEN_SK, 300, 1.0, "bsb", "fp32", 40, 120), (sb.GEN_SK, 500, 1.0, "asb", "split", 3, 90)]
+    os.environ["SB_SOLO_GRAPH"] = "0"
+    try:
+        eager_ctx = sb.Context(0)
+    finally:
+        del os.environ["SB_SOLO_GRAPH"]
+    with sb.Context(0) as ctx:
+        for kind, N, p, v, prec, R, steps in cases:
+            J = sb.generate(kind, N, p, 5)
+            kw = dict(state=True, variant=v, precision=prec, steps=steps, replicas=R, init="uniform", seed=3,
+                      dt=1.0 if v == "asb" else 0.5)
+            pe = eager_ctx.problem(J)
+            ref = eager_ctx.solve(pe, **kw)
+            pg = ctx.problem(J)
+            other = ctx.problem(sb.generate(sb.GEN_KN, 100, 1.0, 9))
+            for rep in range(3):
+                got = ctx.solve(pg, **kw)
+                assert np.array_equal(got.x, ref.x) and np.array_equal(got.y, ref.y), (N, R, rep)
+                assert np.array_equal(got.energy, ref.energy)
+                ctx.solve(other, variant="bsb", precision="fp32", steps=7, replicas=9, init="uniform", seed=rep)
+            pg.close()
+            other.close()
+            pe.close()
+    eager_ctx.close()

@@ -804,11 +804,23 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row0 = (tile * 4 + wave) * ROWS;
     if (row0 >= a.N) return;
+    // Lane q * kThinMaxR + r updates spin row0 + q of replica r; its x, y (and
+    // h) are loaded before the sum so their latency overlaps the J stream.
+    const int oq = lane / kThinMaxR, orr = lane % kThinMaxR, oi = row0 + oq;
+    const bool owner = lane < ROWS * kThinMaxR && orr < a.R && oi < a.N;
+    const size_t o = (size_t)orr * a.ldx + oi;
+    float x = 0.0f, y = 0.0f, hv = 0.0f;
+    if (owner) {
+        x = a.X[o];
+        y = a.Y[o];
+        if (a.h) hv = a.h[oi];
+    }
     float acc[ROWS][kThinMaxR];
 #pragma unroll
     for (int q = 0; q < ROWS; ++q)
 #pragma unroll
         for (int r = 0; r < kThinMaxR; ++r) acc[q][r] = 0.0f;
+#pragma unroll 2
     for (int k0 = 16 * lane; k0 < a.N; k0 += 1024) {
         // k0 + 16 <= round_up(N, 16) <= ld: the group is in bounds.
         float jv[ROWS][16];
@@ -844,19 +856,14 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
         for (int r = 0; r < kThinMaxR; ++r)
 #pragma unroll
             for (int m = 32; m >= 1; m >>= 1) acc[q][r] += __shfl_xor(acc[q][r], m);
-    // Lane q * kThinMaxR + r updates spin row0 + q of replica r.
-    if (lane >= ROWS * kThinMaxR) return;
-    const int q = lane / kThinMaxR, r = lane % kThinMaxR, i = row0 + q;
-    if (r >= a.R || i >= a.N) return;
+    if (!owner) return;
     float F = 0.0f;
 #pragma unroll
     for (int qq = 0; qq < ROWS; ++qq)
 #pragma unroll
         for (int rr = 0; rr < kThinMaxR; ++rr)
-            if (qq == q && rr == r) F = acc[qq][rr];
-    if (a.h) F = F + a.h[i];  // external field, after the sum
-    const size_t o = (size_t)r * a.ldx + i;
-    float x = a.X[o], y = a.Y[o];
+            if (qq == oq && rr == orr) F = acc[qq][rr];
+    if (a.h) F = F + hv;  // external field, after the sum
     sb_update(a.c, F, x, y);
     a.Xn[o] = x;
     a.Y[o] = y;

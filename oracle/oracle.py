@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 ASB, BSB, DSB, ASB_M1 = 0, 1, 2, 3
-ORDER_SEQ, ORDER_LANES = 0, 1  # field summation orders (sb_oracle.h)
+ORDER_SEQ, ORDER_LANES, ORDER_MFMA = 0, 1, 2  # field summation orders (sb_oracle.h)
 
 _lib = None
 
@@ -37,6 +37,7 @@ def load():
             "orc_field_f32": (None, [I, I, P, I, P, I, I, P]),
             "orc_field_f64": (None, [I, I, P, I, P, I, I, P]),
             "orc_field_f32_lanes": (None, [I, I, P, I, P, I, I, P]),
+            "orc_field_f32_mfma": (None, [I, I, P, I, P, I, I, P]),
             "orc_bsb_update_f32": (None, [F, F, F, F, C.POINTER(F), C.POINTER(F)]),
             "orc_asb_local_f32": (None, [C.POINTER(F), C.POINTER(F), F, D, D, D, I]),
             "orc_m1_x_f32": (F, [F, F, D, D]),
@@ -120,7 +121,8 @@ def field_f32(J, x, use_sign=False, order=ORDER_SEQ):
     N = J.shape[0]
     R, ld = x.shape
     F = np.zeros((R, ld), np.float32)
-    fn = load().orc_field_f32 if order == ORDER_SEQ else load().orc_field_f32_lanes
+    fn = {ORDER_SEQ: load().orc_field_f32, ORDER_LANES: load().orc_field_f32_lanes,
+          ORDER_MFMA: load().orc_field_f32_mfma}[order]
     fn(N, R, _p(J), J.shape[1], _p(x), ld, int(use_sign), _p(F))
     return F
 

@@ -208,8 +208,45 @@ static void field_rows_f32_lanes(int N, const float *J, int ldj, const float *xs
     }
 }
 
+/* Position in [0, round_up(N, 32)) of the t-th product of a row's MFMA chain. */
+static inline int mfma_k(int t) { return (t & ~7) + ((t & 1) ? 4 : 0) + ((t & 7) >> 1); }
+
+/* The MFMA order (orc_field_f32_mfma), eight rows at a time (independent
+ * chains, each in its own order); xs_pad holds round_up(N, 32) values. */
+static void field_rows_f32_mfma(int N, const float *J, int ldj, const float *xs_pad, float *F, int row_threads)
+{
+    const int Kp = (N + 31) / 32 * 32;
+    const int nb = (N + 7) / 8;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(row_threads) schedule(static) if (row_threads > 1)
+#endif
+    for (int b = 0; b < nb; ++b) {
+        const int i0 = 8 * b, nr = N - i0 < 8 ? N - i0 : 8;
+        float a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < Kp; ++t) {
+            const int k = mfma_k(t);
+            const float v = xs_pad[k];
+            for (int q = 0; q < nr; ++q) {
+                const float jv = k < N ? J[(size_t)(i0 + q) * ldj + k] : 0.0f;
+                a[q] = fmaf(v, jv, a[q]);
+            }
+        }
+        for (int q = 0; q < nr; ++q) F[i0 + q] = a[q];
+    }
+}
+
 static void field_order(int order, int variant, int N, const float *J, int ldj, const float *x,
                         float *s, float *F, int row_threads);
+
+void orc_field_f32_mfma(int N, int R, const float *J, int ldj, const float *x, int ld,
+                        int use_sign, float *F)
+{
+    float *s = (float *)malloc(sizeof(float) * (size_t)(N + 32));
+    for (int r = 0; r < R; ++r)
+        field_order(ORC_ORDER_MFMA, use_sign ? ORC_DSB : ORC_BSB, N, J, ldj, x + (size_t)r * ld, s,
+                    F + (size_t)r * ld, 1);
+    free(s);
+}
 
 void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, int ld,
                          int use_sign, float *F)
@@ -409,6 +446,15 @@ static void field_order(int order, int variant, int N, const float *J, int ldj, 
         field_rows_f32(N, J, ldj, src, F, row_threads);
         return;
     }
+    if (order == ORC_ORDER_MFMA) {
+        const int Kp32 = (N + 31) / 32 * 32;
+        for (int j = 0; j < Kp32; ++j) {
+            const float v = j < N ? x[j] : 0.0f;
+            s[j] = sign ? sgnf(v) : v;
+        }
+        field_rows_f32_mfma(N, J, ldj, s, F, row_threads);
+        return;
+    }
     for (int j = 0; j < Kp; ++j) {
         const float v = j < N ? x[j] : 0.0f;
         s[j] = sign ? sgnf(v) : v;
@@ -478,14 +524,14 @@ int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int 
 {
     int rc = check_run_args(variant, N, R, ldj, ld, k0, k1, steps, M);
     if (rc) return rc;
-    if (order != ORC_ORDER_SEQ && order != ORC_ORDER_LANES) return ORC_E_INVALID;
+    if (order != ORC_ORDER_SEQ && order != ORC_ORDER_LANES && order != ORC_ORDER_MFMA) return ORC_E_INVALID;
     if (!orc_has_fma()) return ORC_E_NOFMA;
     if (nthreads < 1) nthreads = 1;
-    if (nthreads > 1 && R < nthreads && order == ORC_ORDER_SEQ) {
+    if (nthreads > 1 && R < nthreads && order != ORC_ORDER_LANES) {
         /* Fewer replicas than threads (e.g. R = 1): replicas one after the
          * other, the rows of each field over the threads (same sums). */
         float *F = (float *)malloc(sizeof(float) * (size_t)N);
-        float *s = (float *)malloc(sizeof(float) * (size_t)(N + 16));
+        float *s = (float *)malloc(sizeof(float) * (size_t)(N + 32));
         for (int r = 0; r < R; ++r)
             run_f32_replica(order, variant, N, J, ldj, h, x + (size_t)r * ld, y + (size_t)r * ld,
                             F, s, k0, k1, steps, dt, a0, c0, K, M, nthreads);
@@ -498,7 +544,7 @@ int orc_run_f32_order(int order, int variant, int N, int R, const float *J, int 
 #endif
     {
         float *F = (float *)malloc(sizeof(float) * (size_t)N);
-        float *s = (float *)malloc(sizeof(float) * (size_t)(N + 16));
+        float *s = (float *)malloc(sizeof(float) * (size_t)(N + 32));
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif

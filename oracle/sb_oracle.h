@@ -38,8 +38,10 @@ extern "C" {
 enum { ORC_ASB = 0, ORC_BSB = 1, ORC_DSB = 2, ORC_ASB_M1 = 3 };
 /* Field summation orders: SEQ is the plain j = 0..N-1 fmaf chain per row;
  * LANES regroups it into 64 strided lane sums combined pairwise (the order of
- * the few-replica matrix-vector kernels; see orc_field_f32_lanes). */
-enum { ORC_ORDER_SEQ = 0, ORC_ORDER_LANES = 1 };
+ * the few-replica matrix-vector kernels; see orc_field_f32_lanes); MFMA is one
+ * fmaf chain over a fixed permutation of j (the order of the FP32 MFMA tile
+ * kernels; see orc_field_f32_mfma). */
+enum { ORC_ORDER_SEQ = 0, ORC_ORDER_LANES = 1, ORC_ORDER_MFMA = 2 };
 
 enum {
     ORC_OK = 0,
@@ -95,6 +97,16 @@ void orc_field_f64(int N, int R, const float *J, int ldj, const double *x, int l
  * combined as p[l] = p[l] + p[l ^ m] for m = 32, 16, ..., 1 and p[0] taken. */
 void orc_field_f32_lanes(int N, int R, const float *J, int ldj, const float *x, int ld,
                          int use_sign, float *F);
+
+/* The same products in the MFMA order: one fmaf chain per row over
+ * j in [0, round_up(N, 32)), block of 8 by block of 8, each block in the order
+ * 0, 4, 1, 5, 2, 6, 3, 7 (positions >= N add fmaf(pad, 0, acc), pad = 0 or
+ * sign(0) = -1 for dSB).  v_mfma_f32_32x32x2_f32 applies its two products
+ * to the accumulator as two fmaf, the lane-half-0 product first; the FP32
+ * tiles feed MFMA c (c = 0..3) of each 8-wide k block with k = c (half 0)
+ * and k = 4 + c (half 1). */
+void orc_field_f32_mfma(int N, int R, const float *J, int ldj, const float *x, int ld,
+                        int use_sign, float *F);
 
 /* aSB-m1 (Bifurcation_m1.cu.c:55-92): the x-step and the y-step with the
  * coupling sum F of the new x, in the reference's FP32/FP64 mix. */

@@ -101,15 +101,19 @@ def test_fp32_one_step_within_field_bound(ctx, orc, N, R):
 
 
 def test_fp32_trajectory_statistical_parity(ctx, orc):
-    # Different summation order (MFMA tiles vs sequential fmaf): trajectories may
-    # split where a spin sits near x = 0; most replicas end with identical
-    # spins and the energy distribution is unchanged.
+    # Against the sequential order (the reference's naive kernel) the MFMA
+    # tiles sum in a different order: trajectories may split where a spin sits
+    # near x = 0; most replicas end with identical spins and the energy
+    # distribution is unchanged.
     N, R, steps = 256, 64, 400
     J = kn_pm1(N, 21)
     prob = ctx.problem(J)
     sol = ctx.solve(prob, state=True, variant="bsb", precision="fp32", steps=steps, replicas=R,
                     init="uniform", seed=9, dt=0.5)
     xo, yo, c0 = oracle_run(orc, sb.BSB, J, R, steps, "uniform", 9, 0.5)
+    # bit for bit in the FP32 tiles' own summation order
+    xm, ym, _ = orc.solve_f32(sb.BSB, J, R, steps, init="uniform", seed=9, dt=0.5, order=orc.ORDER_MFMA)
+    assert np.array_equal(sol.x, xm[:, :N]) and np.array_equal(sol.y, ym[:, :N])
     so = orc.spins(np.ascontiguousarray(xo), N)
     same = np.all(sol.spins == so, axis=1)
     Ho = orc.energy_i64(J, so)

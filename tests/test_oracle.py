@@ -172,12 +172,15 @@ def test_field_f32_error_bound(orc):
     assert np.all(np.abs(F32 - F64) <= bound)
 
 
-def test_lane_order_field_pins(orc):
+@pytest.mark.parametrize("order", ["lanes", "mfma"])
+def test_reordered_field_pins(orc, order):
     # Integer couplings times integer-valued state are exact in any order, so
-    # the lane-order field must equal the integer product: a dropped, repeated
-    # or misplaced 16-group or a wrong butterfly step fails.  The sizes cover
-    # one partial group, whole lane sweeps and partial last sweeps.
-    for N in (5, 16, 70, 1023, 1025, 2100):
+    # the lane-order and MFMA-order fields must equal the integer product: a
+    # dropped, repeated or misplaced group, a wrong butterfly step or a
+    # permutation that misses a position fails.  The sizes cover one partial
+    # group, whole lane sweeps, partial last sweeps and partial 8- and 32-blocks.
+    ORD = orc.ORDER_LANES if order == "lanes" else orc.ORDER_MFMA
+    for N in (5, 16, 33, 70, 1023, 1025, 2100):
         J = RNG.integers(-16, 17, (N, N)).astype(np.float32)
         J = np.triu(J, 1)
         J = J + J.T
@@ -186,10 +189,10 @@ def test_lane_order_field_pins(orc):
         xi[:, :N] = RNG.integers(-3, 4, (3, N))
         x = xi.astype(np.float32)
         exact = xi[:, :N] @ J.astype(np.int64).T
-        F = orc.field_f32(J, x, order=orc.ORDER_LANES)
+        F = orc.field_f32(J, x, order=ORD)
         assert np.array_equal(F[:, :N].astype(np.int64), exact), N
         s = np.where(x[:, :N] > 0, 1, -1).astype(np.int64)
-        Fs = orc.field_f32(J, x, use_sign=True, order=orc.ORDER_LANES)
+        Fs = orc.field_f32(J, x, use_sign=True, order=ORD)
         assert np.array_equal(Fs[:, :N].astype(np.int64), s @ J.astype(np.int64).T), N
     # Real couplings: within the FP32 bound of the FP64 field, and a different
     # rounding sequence from the sequential chain.
@@ -198,7 +201,7 @@ def test_lane_order_field_pins(orc):
     J = np.triu(J, 1)
     J = J + J.T
     x, _ = orc.init_state("uniform", N, 3, seed=4)
-    Fl = orc.field_f32(J, x, order=orc.ORDER_LANES)[:, :N]
+    Fl = orc.field_f32(J, x, order=ORD)[:, :N]
     Fq = orc.field_f32(J, x)[:, :N]
     F64 = orc.field_f64(J, x.astype(np.float64))[:, :N]
     bound = N * 2.0**-24 * (np.abs(x[:, :N]).astype(np.float64) @ np.abs(J).T.astype(np.float64))
@@ -206,16 +209,43 @@ def test_lane_order_field_pins(orc):
     assert np.any(Fl != Fq)
 
 
-def test_lane_order_run_equals_sequential_for_exact_fields(orc):
-    # dSB with integer J: every field is an exact integer, so the two orders
-    # give the same trajectory bit for bit.
+@pytest.mark.parametrize("order", ["lanes", "mfma"])
+def test_reordered_run_equals_sequential_for_exact_fields(orc, order):
+    # dSB with integer J: every field is an exact integer, so the orders give
+    # the same trajectory bit for bit.
+    ORD = orc.ORDER_LANES if order == "lanes" else orc.ORDER_MFMA
     N, R, steps = 300, 4, 60
     J = RNG.integers(-2, 3, (N, N)).astype(np.float32)
     J = np.triu(J, 1)
     J = J + J.T
     xa, ya, _ = orc.solve_f32(orc.DSB, J, R, steps, seed=5, dt=0.5)
-    xb, yb, _ = orc.solve_f32(orc.DSB, J, R, steps, seed=5, dt=0.5, order=orc.ORDER_LANES)
+    xb, yb, _ = orc.solve_f32(orc.DSB, J, R, steps, seed=5, dt=0.5, order=ORD)
     assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+
+
+def test_mfma_order_chain_on_crafted_rows(orc):
+    # The MFMA order is one fmaf chain over the positions 0,4,1,5,2,6,3,7 of
+    # each 8-block.  A row whose only nonzero products sit at k = 1 and k = 4
+    # with values that round differently in the two orders: 1 + 2^-24 + 2^-24
+    # is 1 in either fmaf chain that adds them one at a time, while a row with
+    # a large product at k = 4 first and a cancelling one at k = 1 keeps the
+    # rounding error of the k = 4 product only if k = 4 comes first.
+    N = 8
+    J = np.zeros((N, N), np.float32)
+    a = np.float32(1.0 + 2.0**-23)
+    J[0, 4] = a
+    J[0, 1] = -a
+    J[4, 0] = a
+    J[1, 0] = -a
+    x = np.zeros((1, 64), np.float32)
+    x[0, 4] = a
+    x[0, 1] = a
+    F = orc.field_f32(J, x, order=orc.ORDER_MFMA)
+    # k = 4 first: fmaf(a, a, 0) = fl(a^2); then fmaf(a, -a, fl(a^2)) = fl(a^2) - a^2 = -2^-46
+    assert F[0, 0] == np.float32(-(2.0**-46))
+    Fs = orc.field_f32(J, x, order=orc.ORDER_SEQ)
+    # sequential: k = 1 first gives +2^-46
+    assert Fs[0, 0] == np.float32(2.0**-46)
 
 
 # ---- R12 aSB-m1 (Bifurcation_m1.cu.c:55-92) ------------------------------------

@@ -149,33 +149,45 @@ def test_thin_sb_step_matches_solve(orc):
         prob.close()
 
 
+def _ctx_with(env):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return sb.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
 def test_solo_graph_replays_equal_eager_launches(orc):
     # Launch-bound solves (R <= 4, or small MFMA steps) replay their step loop
     # from a cached graph.  Every replay, with other eager solves and
     # synchronisations in between, must equal the step-by-step launches.
-    import os
     cases = [(sb.GEN_ER, 800, 0.06, "bsb", "fp32", 1, 300), (sb.GEN_KN, 2000, 1.0, "dsb", "split", 2, 150),
-             (sb.GEN_SK, 300, 1.0, "bsb", "fp32", 40, 120), (sb.GEN_SK, 500, 1.0, "asb", "split", 3, 90)]
-    os.environ["SB_SOLO_GRAPH"] = "0"
-    try:
-        eager_ctx = sb.Context(0)
-    finally:
-        del os.environ["SB_SOLO_GRAPH"]
+             (sb.GEN_SK, 300, 1.0, "bsb", "fp32", 40, 120), (sb.GEN_SK, 500, 1.0, "asb", "split", 3, 90),
+             (sb.GEN_SK, 3000, 1.0, "asb_m1", "fp32", 4, 33)]
+    eager = _ctx_with({"SB_SOLO_GRAPH": "0"})
     with sb.Context(0) as ctx:
         for kind, N, p, v, prec, R, steps in cases:
             J = sb.generate(kind, N, p, 5)
+            h = np.random.default_rng(N).standard_normal(N).astype(np.float32) if N in (800, 3000) else None
             kw = dict(state=True, variant=v, precision=prec, steps=steps, replicas=R, init="uniform", seed=3,
                       dt=1.0 if v == "asb" else 0.5)
-            pe = eager_ctx.problem(J)
-            ref = eager_ctx.solve(pe, **kw)
-            pg = ctx.problem(J)
+            pe = eager.problem(J, h)
+            ref = eager.solve(pe, **kw)
+            pg = ctx.problem(J, h)
             other = ctx.problem(sb.generate(sb.GEN_KN, 100, 1.0, 9))
             for rep in range(3):
                 got = ctx.solve(pg, **kw)
-                assert np.array_equal(got.x, ref.x) and np.array_equal(got.y, ref.y), (N, R, rep)
-                assert np.array_equal(got.energy, ref.energy)
+                assert np.array_equal(got.x, ref.x, equal_nan=True), (N, R, rep)
+                assert np.array_equal(got.y, ref.y, equal_nan=True), (N, R, rep)
+                assert np.array_equal(got.energy, ref.energy, equal_nan=True)
                 ctx.solve(other, variant="bsb", precision="fp32", steps=7, replicas=9, init="uniform", seed=rep)
             pg.close()
             other.close()
             pe.close()
-    eager_ctx.close()
+    eager.close()

@@ -264,13 +264,17 @@ int Engine::rebuild()
         I.c0 = s.c0;
         I.K = s.params.K;
         if (!I.active) continue;
-        // Cost model: MFMA tiles by their area; a thin tile streams its rows of
-        // J on the VALU at roughly the cost of a 64-replica MFMA row block.
-        // A sparse tile gathers 4 spins' nonzeros for 64 replicas, at about the
-        // cost of 4 MFMA tile rows per nonzero.
+        // Cost model: MFMA tiles by their area times N; a thin tile streams its
+        // rows of J on the VALU at roughly the cost of a 64-replica MFMA row
+        // block.  A sparse tile is 4 waves, each a chain of dependent gathers
+        // over one row's nonzeros (a few loads in flight per wave): estimated
+        // at ~50 ns per nonzero, i.e. ~5e4 area-N units of a split-precision
+        // MFMA tile (~1024 units per ns per workgroup at two workgroups per
+        // CU) and ~8x fewer units of the FP32 MFMA rate.
+        const double gather_units = s.params.precision == SB_PREC_FP32 ? 6400.0 : 51200.0;
         const double tile_cost =
             s.family == FAM_SPARSE
-                ? 4.0 * 64.0 * 4.0 * std::max(1.0, (double)s.prob->nnz / N)
+                ? gather_units * std::max(1.0, (double)s.prob->nnz / N)
                 : (double)sh.mt() * (sh.rows ? 64 : sh.nt()) * (double)round_up(N, kSpinPad);
         for (int t = 0; t < I.tiles_m * I.tiles_n; ++t) per_slot[i].push_back({tile_cost, (int)i, t});
         slot_cost[i] = tile_cost * I.tiles_m * I.tiles_n;

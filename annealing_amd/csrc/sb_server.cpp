@@ -1,8 +1,12 @@
 // sb_server.cpp -- continuous-batching request server.
 //
 // A worker thread owns one Engine (device-resident J and state per request).
-// An instance with at most 2 % nonzeros runs the sparse family (its CSR form
-// is built on the device at admission).
+// With sparse_density > 0, an instance with at most that fraction of nonzeros
+// runs the sparse family (its CSR form is built on the device at admission).
+// It is off by default: inside the persistent launch (two workgroups per CU)
+// a sparse work item is bound by the latency of its dependent gathers, and
+// in a mixed load where 1 % of the requests went sparse the served rate fell
+// threefold.
 // Between chunks of `chunk_steps` steps it retires finished requests (energies
 // and spins read back, device memory freed) and admits queued requests while
 // the resident set stays within two bounds: the HBM budget (J + state of every
@@ -92,8 +96,9 @@ struct sb_server {
     std::vector<uint64_t> active;
     size_t bytes_in_use = 0;
     long long active_tiles = 0, tile_budget = 0;
-    // Instances with at most this fraction of nonzeros run the sparse family.
-    double sparse_density = 0.02;
+    // Instances with at most this fraction of nonzeros run the sparse family
+    // (0: never).
+    double sparse_density = 0.0;
     sb_server_stats stats{};
 
     void run();
@@ -272,9 +277,11 @@ extern "C" SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, 
     if (!ctx || !opts || !out || opts->hbm_budget_bytes == 0 || opts->chunk_steps < 1) return SB_E_INVALID;
     if (opts->precision != SB_PREC_FP32 && opts->precision != SB_PREC_SPLIT) return SB_E_INVALID;
     if (opts->policy != SB_SERVE_SHORTEST && opts->policy != SB_SERVE_ARRIVAL) return SB_E_INVALID;
+    if (!(opts->sparse_density >= 0.0 && opts->sparse_density <= 1.0)) return SB_E_INVALID;
     sb_server *s = new sb_server();
     s->ctx = ctx;
     s->opts = *opts;
+    s->sparse_density = opts->sparse_density;
     {
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         if (hipSetDevice(ctx->device) != hipSuccess) {
@@ -288,7 +295,6 @@ extern "C" SB_API int sb_server_create(sb_ctx *ctx, const sb_server_opts *opts, 
         }
         s->tile_budget = 2LL * s->engine->grid();
         if (const char *tb = std::getenv("SB_SERVE_TILE_BUDGET")) s->tile_budget = std::max(1LL, std::atoll(tb));
-        if (const char *sd = std::getenv("SB_SERVE_SPARSE_DENSITY")) s->sparse_density = std::atof(sd);
     }
     s->worker = std::thread([s] { s->run(); });
     *out = s;

@@ -96,6 +96,7 @@ class ServerOpts(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_size_t), ("chunk_steps", C.c_int32),
         ("max_active", C.c_int32), ("precision", C.c_int32), ("policy", C.c_int32),
+        ("sparse_density", C.c_double),
     ]
 
 
@@ -433,7 +434,7 @@ class Server:
     """Continuous-batching request server (sb_server_* of include/sb.h)."""
 
     def __init__(self, ctx: Context, hbm_budget_bytes: int, chunk_steps: int = 50, max_active: int = 0,
-                 precision="split", policy="shortest"):
+                 precision="split", policy="shortest", sparse_density: float = 0.0):
         self.ctx = ctx
         self.lib = ctx.lib
         o = ServerOpts()
@@ -442,6 +443,7 @@ class Server:
         o.max_active = max_active
         o.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
         o.policy = SERVE_POLICIES[policy] if isinstance(policy, str) else policy
+        o.sparse_density = float(sparse_density)
         h = C.c_void_p()
         _check(self.lib.sb_server_create(ctx.handle, C.byref(o), C.byref(h)), "sb_server_create")
         self.handle = h

@@ -237,9 +237,15 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
     done_evt = threading.Event()
     lock = threading.Lock()
 
-    def waiter():
-        last = time.perf_counter()
-        for i in range(n_req):
+    # Requests retire out of submission order (shortest work first), so several
+    # threads collect them, each blocking on its own share of the tickets in
+    # turn; the latency is stamped by the server when a request retires, not
+    # when it is collected here.
+    n_wait = 32
+    progress = {"retired": 0, "last": time.perf_counter()}
+
+    def waiter(w):
+        for i in range(w, n_req, n_wait):
             while tickets[i] is None:
                 time.sleep(0.001)
             try:
@@ -248,13 +254,19 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
                     lat[i] = r.total_ms
             except sb.SBError:
                 pass
-            if time.perf_counter() - last > 30.0:  # long windows: report progress as requests finish
-                last = time.perf_counter()
-                log(f"serving: {i + 1}/{n_req} requests retired")
-        done_evt.set()
+            with lock:
+                progress["retired"] += 1
+                retired = progress["retired"]
+                report = time.perf_counter() - progress["last"] > 30.0
+                if report:
+                    progress["last"] = time.perf_counter()
+                if retired == n_req:
+                    done_evt.set()
+            if report:  # long windows: report progress as requests retire
+                log(f"serving: {retired}/{n_req} requests retired")
 
-    th = threading.Thread(target=waiter, daemon=True)
-    th.start()
+    for w in range(n_wait):
+        threading.Thread(target=waiter, args=(w,), daemon=True).start()
     t0 = time.perf_counter()
     for i in range(n_req):
         delay = arrivals[i] - (time.perf_counter() - t0)

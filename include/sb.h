@@ -304,6 +304,8 @@ typedef struct sb_server_opts {
     int32_t max_active;       /* cap on concurrently resident requests (0: none) */
     int32_t precision;        /* sb_precision for all requests                   */
     int32_t policy;           /* sb_serve_policy                                 */
+    double sparse_density;    /* requests with at most this fraction of nonzero
+                                 couplings run the sparse (CSR) family; 0: never */
 } sb_server_opts;
 
 typedef struct sb_server_stats {

@@ -111,12 +111,13 @@ def test_sparse_problems_in_a_batch(ctx, orc, precision):
 
 
 def test_server_runs_low_density_requests_sparse(ctx, orc):
-    # At most 2 % nonzeros: the server builds the CSR form at admission and the
-    # request follows the oracle's sequential order (bSB on real J included).
+    # sparse_density=0.02: a request with at most 2 % nonzeros gets its CSR form
+    # built at admission and follows the oracle's sequential order (bSB on real
+    # J included).
     GB = 1 << 30
     specs = [(sb.GEN_SK_DILUTED, 2000, 0.01, 31, "bsb", 64, 120), (sb.GEN_ER, 700, 0.015, 32, "dsb", 33, 90),
              (sb.GEN_ER, 500, 0.5, 33, "bsb", 32, 60)]
-    with sb.Server(ctx, 8 * GB, chunk_steps=16, precision="split") as srv:
+    with sb.Server(ctx, 8 * GB, chunk_steps=16, precision="split", sparse_density=0.02) as srv:
         tickets = [srv.submit_generated(k, N, p, g, variant=v, precision="split", steps=s, replicas=R,
                                         init="uniform", seed=g, dt=0.5) for k, N, p, g, v, R, s in specs]
         res = [srv.wait(t) for t in tickets]

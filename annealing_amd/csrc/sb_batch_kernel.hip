@@ -151,13 +151,13 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 0: dsb_fp8_tile<2, 2, 1, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 1: dsb_fp8_tile<2, 2, 2, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 2: dsb_fp8_tile<4, 1, 1, 1, true>(a, item.tile, smem_dyn); break;
-#define SB_THIN_CASES(fam, jb)                                                                  \
-        case (fam * 2 + 0) * 8 + 0: thin_tile<1, jb, false>(a, item.tile); break;              \
-        case (fam * 2 + 0) * 8 + 1: thin_tile<2, jb, false>(a, item.tile); break;              \
-        case (fam * 2 + 0) * 8 + 2: thin_tile<4, jb, false>(a, item.tile); break;              \
-        case (fam * 2 + 1) * 8 + 0: thin_tile<1, jb, true>(a, item.tile); break;               \
-        case (fam * 2 + 1) * 8 + 1: thin_tile<2, jb, true>(a, item.tile); break;               \
-        case (fam * 2 + 1) * 8 + 2: thin_tile<4, jb, true>(a, item.tile); break;
+#define SB_THIN_CASES(fam, jb)                                                                          \
+        case (fam * 2 + 0) * 8 + 0: thin_tile<1, jb, false, kThinMaxR>(a, item.tile); break;            \
+        case (fam * 2 + 0) * 8 + 1: thin_tile<2, jb, false, kThinMaxR>(a, item.tile); break;            \
+        case (fam * 2 + 0) * 8 + 2: thin_tile<4, jb, false, kThinMaxR>(a, item.tile); break;            \
+        case (fam * 2 + 1) * 8 + 0: thin_tile<1, jb, true, kThinMaxR>(a, item.tile); break;             \
+        case (fam * 2 + 1) * 8 + 1: thin_tile<2, jb, true, kThinMaxR>(a, item.tile); break;             \
+        case (fam * 2 + 1) * 8 + 2: thin_tile<4, jb, true, kThinMaxR>(a, item.tile); break;
         SB_THIN_CASES(FAM_THIN_F32, 4)
         SB_THIN_CASES(FAM_THIN_BF16, 2)
         SB_THIN_CASES(FAM_THIN_FP8, 1)

@@ -88,7 +88,7 @@ hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStrea
         a.tiles_m = (a.N + sh.mt() - 1) / sh.mt();
         a.tiles_n = 1;
         a.rpg = 1;
-        StepKernel k = kThinKernels[family - FAM_THIN_F32][a.c.variant == SB_DSB][shape_idx];
+        StepKernel k = kThinKernels[family - FAM_THIN_F32][a.c.variant == SB_DSB][thin_rm_index(a.R)][shape_idx];
         hipLaunchKernelGGL(k, dim3(a.tiles_m), dim3(sh.threads()), 0, s, a);
         return hipGetLastError();
     }

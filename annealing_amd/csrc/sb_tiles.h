@@ -762,22 +762,59 @@ __device__ __forceinline__ void dsb_fp8_tile(const StepArgs &a, int tile, unsign
 // on ROWS nor on how J is stored, so every thin shape, and the FP32, BF16 and
 // FP8 forms of the same J values, give the same field.
 
+// The xor butterfly over the 64 lanes, levels 32, 16, 8, 4, 2, 1: at level m
+// lane l adds lane l ^ m's value to its own.  Level 32 goes through
+// ds_bpermute, levels 16..4 through ds_swizzle (xor mask within 32 lanes) and
+// levels 2, 1 through DPP quad permutes; each level is the same addition as
+// v + __shfl_xor(v, m).
+template <int M>
+__device__ __forceinline__ float xor_lane(float v)
+{
+    const int b = __builtin_bit_cast(int, v);
+    int p;
+    if constexpr (M == 1)
+        p = __builtin_amdgcn_update_dpp(b, b, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+    else if constexpr (M == 2)
+        p = __builtin_amdgcn_update_dpp(b, b, 0x4E, 0xF, 0xF, false);  // quad_perm [2, 3, 0, 1]
+    else if constexpr (M < 32)
+        p = __builtin_amdgcn_ds_swizzle(b, 0x1F | (M << 10));  // and 0x1F, or 0, xor M
+    else
+        return __shfl_xor(v, M);
+    return __builtin_bit_cast(float, p);
+}
+__device__ __forceinline__ float wave_sum_xor(float v)
+{
+    v += xor_lane<32>(v);
+    v += xor_lane<16>(v);
+    v += xor_lane<8>(v);
+    v += xor_lane<4>(v);
+    v += xor_lane<2>(v);
+    v += xor_lane<1>(v);
+    return v;
+}
+
+// One lane's 16 consecutive J values of a row, as stored (FP32: 4 x 16 bytes,
+// BF16: 2, FP8: 1), and their widening to FP32 (exact for BF16 and FP8 E4M3).
 template <int JB>
-__device__ __forceinline__ void thin_load_j(const void *J, size_t o, float (&v)[16])
+struct ThinJRaw {
+    u32x4 w[JB];
+};
+template <int JB>
+__device__ __forceinline__ void thin_widen_j(const ThinJRaw<JB> &raw, float (&v)[16])
 {
     if constexpr (JB == 4) {
-        const v4f *p = (const v4f *)((const float *)J + o);
+        // Whole-vector bit casts: hipcc (ROCm 7.2) folds a bit cast of a
+        // single vector element, bit_cast<float>(w[e]), to element 0.
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            const v4f t = p[g];
+            const v4f t = __builtin_bit_cast(v4f, raw.w[g]);
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[4 * g + e] = t[e];
         }
     } else if constexpr (JB == 2) {
-        const u32x4 *p = (const u32x4 *)((const unsigned short *)J + o);
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
-            const u32x4 t = p[g];
+            const u32x4 t = raw.w[g];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 v[8 * g + 2 * e] = __builtin_bit_cast(float, t[e] << 16);
@@ -785,7 +822,7 @@ __device__ __forceinline__ void thin_load_j(const void *J, size_t o, float (&v)[
             }
         }
     } else {
-        const u32x4 t = *(const u32x4 *)((const unsigned char *)J + o);
+        const u32x4 t = raw.w[0];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             unsigned int lo2, hi2;  // BF16 pairs of bytes 0,1 and 2,3
@@ -798,10 +835,28 @@ __device__ __forceinline__ void thin_load_j(const void *J, size_t o, float (&v)[
     }
 }
 
-template <int ROWS, int JB, bool SGN>
+// One group (16 consecutive k at k0) of every row and replica of a wave, as
+// loaded.  Replica slots r >= R (RM = 4 with R = 3) load replica 0 again and
+// their sums are never used.
+template <int ROWS, int JB, int RM>
+struct ThinGroup {
+    ThinJRaw<JB> j[ROWS];
+    u32x4 x[RM][4];
+};
+
+// RM: replicas per kernel (1, 2 or 4; R <= RM).  Every load is a range-checked
+// buffer load (a row or replica holds round_up(N, 16) in-range elements, the
+// rest read as zeros), so the loads carry no branch: with RM a compile-time
+// constant a group's loads issue together, and when two groups fit the
+// registers (PIPE) the loads of group it + 1 issue before group it is summed,
+// so a row of up to 2048 spins takes one memory round trip and a longer row
+// streams with a group in flight.  A lane sums a group only if it starts below
+// N, exactly as above.
+template <int ROWS, int JB, bool SGN, int RM>
 __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wave (hence row0 and the buffer resources) is uniform across the wave.
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int row0 = (tile * 4 + wave) * ROWS;
     if (row0 >= a.N) return;
     // Lane q * kThinMaxR + r updates spin row0 + q of replica r; its x, y (and
@@ -815,53 +870,84 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
         y = a.Y[o];
         if (a.h) hv = a.h[oi];
     }
-    float acc[ROWS][kThinMaxR];
+    float acc[ROWS][RM];
 #pragma unroll
     for (int q = 0; q < ROWS; ++q)
 #pragma unroll
-        for (int r = 0; r < kThinMaxR; ++r) acc[q][r] = 0.0f;
-#pragma unroll 2
-    for (int k0 = 16 * lane; k0 < a.N; k0 += 1024) {
-        // k0 + 16 <= round_up(N, 16) <= ld: the group is in bounds.
-        float jv[ROWS][16];
+        for (int r = 0; r < RM; ++r) acc[q][r] = 0.0f;
+    // round_up(N, 16) <= ld elements per row and replica are in range (the
+    // padding past N is zero); rows past N have no range and read zeros.
+    const int n16 = (a.N + 15) & ~15;
+    __amdgpu_buffer_rsrc_t rj[ROWS], rx[RM];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q)
+        rj[q] = tile_rsrc((const unsigned char *)a.J + (size_t)(row0 + q) * a.ldj * JB,
+                          row0 + q < a.N ? (long long)n16 * JB : 0);
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+        rx[r] = tile_rsrc(a.X + (size_t)(RM == 1 || r < a.R ? r : 0) * a.ldx, (long long)n16 * 4);
+    using G = ThinGroup<ROWS, JB, RM>;
+    auto load = [&](int k0, G &g) {
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q)
+#pragma unroll
+            for (int e = 0; e < JB; ++e) g.j[q].w[e] = rload16(rj[q], (unsigned)(k0 * JB + 16 * e));
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) g.x[r][e] = rload16(rx[r], (unsigned)(4 * k0 + 16 * e));
+    };
+    auto sum = [&](const G &g) {
+        float xv[RM][16];
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const v4f t = __builtin_bit_cast(v4f, g.x[r][e]);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) xv[r][4 * e + c] = SGN ? sgn1(t[c]) : t[c];
+            }
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) {
-            if (row0 + q < a.N) {
-                thin_load_j<JB>(a.J, (size_t)(row0 + q) * a.ldj + k0, jv[q]);
-            } else {
+            float jv[16];
+            thin_widen_j<JB>(g.j[q], jv);
 #pragma unroll
-                for (int e = 0; e < 16; ++e) jv[q][e] = 0.0f;
-            }
+            for (int r = 0; r < RM; ++r)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[q][r] = __builtin_fmaf(jv[e], xv[r][e], acc[q][r]);
         }
-#pragma unroll
-        for (int r = 0; r < kThinMaxR; ++r) {
-            if (r >= a.R) break;
-            const v4f *xp = (const v4f *)(a.X + (size_t)r * a.ldx + k0);
-            float xv[16];
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const v4f t = xp[g];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) xv[4 * g + e] = SGN ? sgn1(t[e]) : t[e];
-            }
-#pragma unroll
-            for (int q = 0; q < ROWS; ++q)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) acc[q][r] = __builtin_fmaf(jv[q][e], xv[e], acc[q][r]);
+    };
+    constexpr bool PIPE = 2 * (ROWS * 4 * JB + RM * 16) <= 192;
+    const int groups = (a.N + 1023) >> 10;  // wave-uniform
+    int k0 = 16 * lane;
+    if constexpr (PIPE) {
+        G ga, gb;
+        load(k0, ga);
+        for (int it = 0; it < groups; it += 2) {
+            load(k0 + 1024, gb);
+            if (k0 < a.N) sum(ga);
+            if (it + 1 >= groups) break;
+            load(k0 + 2048, ga);
+            if (k0 + 1024 < a.N) sum(gb);
+            k0 += 2048;
+        }
+    } else {
+        for (int it = 0; it < groups; ++it, k0 += 1024) {
+            G g;
+            load(k0, g);
+            if (k0 < a.N) sum(g);
         }
     }
 #pragma unroll
     for (int q = 0; q < ROWS; ++q)
 #pragma unroll
-        for (int r = 0; r < kThinMaxR; ++r)
-#pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) acc[q][r] += __shfl_xor(acc[q][r], m);
+        for (int r = 0; r < RM; ++r) acc[q][r] = wave_sum_xor(acc[q][r]);
     if (!owner) return;
     float F = 0.0f;
 #pragma unroll
     for (int qq = 0; qq < ROWS; ++qq)
 #pragma unroll
-        for (int rr = 0; rr < kThinMaxR; ++rr)
+        for (int rr = 0; rr < RM; ++rr)
             if (qq == oq && rr == orr) F = acc[qq][rr];
     if (a.h) F = F + hv;  // external field, after the sum
     sb_update(a.c, F, x, y);
@@ -934,7 +1020,9 @@ extern const StepKernel kSplitBf16Kernels[3][kNumSplitShapes];
 extern const StepKernel kSplitFp8Kernels[3][kNumSplitShapes];
 // [sign source: X, plane][shape]
 extern const StepKernel kDsbFp8Kernels[2][kNumFp8Shapes];
-// [J format: FP32, BF16, FP8][sign x][shape]
-extern const StepKernel kThinKernels[3][2][kNumThinShapes];
+// [J format: FP32, BF16, FP8][sign x][replicas per kernel: 1, 2, 4][shape]
+constexpr int kNumThinRm = 3;
+__host__ __device__ constexpr int thin_rm_index(int R) { return R <= 1 ? 0 : R == 2 ? 1 : 2; }
+extern const StepKernel kThinKernels[3][2][kNumThinRm][kNumThinShapes];
 
 }  // namespace sbk

@@ -927,7 +927,11 @@ int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int varian
             return SB_OK;
         }
     }
-    auto key = std::make_tuple(fams[0] * 8 + precision, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
+    // The key holds the whole candidate set: a problem without an FP8 copy of
+    // J must never reuse a choice made among FP8 and BF16 families.
+    int fam_set = 0;
+    for (int f : fams) fam_set |= 1 << f;
+    auto key = std::make_tuple(fam_set * 8 + precision, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
     auto it = ctx->tune.find(key);
     if (it != ctx->tune.end()) {
         *family = it->second.first;
@@ -951,8 +955,9 @@ int choose_kernel(sb_ctx *ctx, const sb_problem *prob, int precision, int varian
 int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int variant, int R, int *bshape)
 {
     const int N = prob->N;
-    // Cache key: family + 100 marks the batched-shape table.
-    auto key = std::make_tuple(family + 100, bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
+    // Cache key: -(family + 1) marks the batched-shape table (the solo keys
+    // are non-negative).
+    auto key = std::make_tuple(-(family + 1), bucket(N), bucket(R), variant == SB_DSB ? 1 : 0);
     auto it = ctx->tune.find(key);
     if (it != ctx->tune.end()) {
         *bshape = it->second.second;

@@ -81,6 +81,8 @@ int rows_per_group(int tiles_m, int tiles_n, int mt, int nt, int bytes_j, int by
 hipError_t launch_step(int family, int shape_idx, const StepArgs &a_in, hipStream_t s)
 {
     if (shape_idx < 0 || shape_idx >= shape_count(family)) return hipErrorInvalidValue;
+    // A family whose form of J the problem does not hold is never launched.
+    if (!a_in.J || !a_in.X || !a_in.Xn || !a_in.Y) return hipErrorInvalidValue;
     TileShape sh = shape_at(family, shape_idx);
     StepArgs a = a_in;
     if (is_thin(family)) {

@@ -295,3 +295,26 @@ def test_cli_fractional_field_writes_fp64_energies(ctx, orc):
     assert np.any(H != np.round(H))
     W = -np.triu(J.astype(np.int64), 1)
     assert np.array_equal(cut, [W[so[r][:, None] != so[r][None, :]].sum() for r in range(4)])
+
+
+def test_split_choice_not_shared_between_fp8_and_bf16_problems():
+    # The autotuner caches its (family, shape) choice per size bucket.  An
+    # FP8-exact problem chooses among the FP8- and BF16-J split families; a
+    # real-valued problem of the same size has no FP8 copy of J, so it must
+    # make its own choice (the cache key holds the candidate set) and give the
+    # result of a fresh context.
+    N, R, steps = 600, 96, 40
+    kw = dict(variant="bsb", precision="split", steps=steps, replicas=R, init="uniform", seed=5, dt=0.5)
+    with sb.Context(0) as ctx:
+        p_int = ctx.problem(sb.generate(sb.GEN_KN, N, 1.0, seed=5))
+        p_real = ctx.problem(sb.generate(sb.GEN_SK, N, 1.0, seed=5))
+        assert p_int.split_jfmt == sb.J_FP8 and p_real.split_jfmt == sb.J_BF16
+        ctx.solve(p_int, **kw)
+        after = ctx.solve(p_real, **kw)
+        p_int.close()
+        p_real.close()
+    with sb.Context(0) as ctx:
+        p_real = ctx.problem(sb.generate(sb.GEN_SK, N, 1.0, seed=5))
+        fresh = ctx.solve(p_real, **kw)
+        p_real.close()
+    assert np.array_equal(after.spins, fresh.spins) and np.array_equal(after.energy, fresh.energy)

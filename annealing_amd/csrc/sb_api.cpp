@@ -977,20 +977,23 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
     // parallelism across instances, so the shape is the one with the least
     // modelled traffic (J read once per replica tile, the B operand once per
     // row tile), not the fastest solo launch; ties go to the larger tile.
-    // Results do not depend on the shape.
+    // Shapes wider than R rounded up to 64 replicas are skipped (their
+    // padding columns would be multiplied for nothing).  Results do not depend
+    // on the shape.
     const double bj = family == FAM_FP32 || family == FAM_THIN_F32 ? 4.0
                       : family == FAM_SPLIT_BF16 || family == FAM_THIN_BF16 ? 2.0 : 1.0;
     const int bk = plane_kind(family, variant);
     const double bb = bk == BK_NONE ? 4.0 : (double)plane_bytes(bk);
-    int pick = 0;
+    int pick = -1;
     double best_cost = 0.0;
     long best_area = 0;
     for (int b = 0; b < nb; ++b) {
         const TileShape sh = batch_shape_at(family, b);
+        if (b > 0 && sh.nt() > (R + 63) / 64 * 64) continue;
         const double tm = (N + sh.mt() - 1) / sh.mt(), tn = (R + sh.nt() - 1) / sh.nt();
         const double cost = tn * (double)N * N * bj + tm * (double)N * R * bb;
         const long area = (long)sh.mt() * sh.nt();
-        if (b == 0 || cost < best_cost || (cost == best_cost && area > best_area)) {
+        if (pick < 0 || cost < best_cost || (cost == best_cost && area > best_area)) {
             pick = b;
             best_cost = cost;
             best_area = area;

@@ -62,15 +62,23 @@ def test_every_shape_same_result(ctx, variant):
                 assert np.array_equal(sol.y, ref.y, equal_nan=True), f"{prec} family {fam} shape {s}"
 
 
-@pytest.mark.parametrize("precision,variant", [("fp32", "bsb"), ("split", "bsb"), ("split", "dsb")])
-def test_every_batch_shape_equals_solo(ctx, precision, variant):
-    Js = [sb.generate(sb.GEN_KN, n, 1.0, 40 + n) for n in (N, 77)]
+@pytest.mark.parametrize("precision,variant,kind", [("fp32", "bsb", sb.GEN_KN), ("split", "bsb", sb.GEN_KN),
+                                                    ("split", "dsb", sb.GEN_KN), ("split", "bsb", sb.GEN_SK),
+                                                    ("split", "dsb", sb.GEN_SK)])
+def test_every_batch_shape_equals_solo(ctx, precision, variant, kind):
+    # K_N (integer J) runs the FP8-J families, SK (real J) the BF16-J family.
+    Js = [sb.generate(kind, n, 1.0, 40 + n) for n in (N, 77)]
     probs = [ctx.problem(J) for J in Js]
     params = [sb.make_params(variant=variant, precision=precision, steps=STEPS, replicas=r, init="uniform", seed=5,
                              dt=0.5) for r in (R, 9)]
     solos = [ctx.solve(p, state=True, variant=variant, precision=precision, steps=STEPS, replicas=r,
                        init="uniform", seed=5, dt=0.5) for p, r in zip(probs, (R, 9))]
-    fam = 0 if precision == "fp32" else (3 if variant == "dsb" else 2)
+    if precision == "fp32":
+        fam = 0
+    elif kind == sb.GEN_SK:
+        fam = 1
+    else:
+        fam = 3 if variant == "dsb" else 2
     for b in range(BATCH_SHAPES[fam]):
         os.environ["SB_BATCH_SHAPE"] = str(b)
         try:

@@ -377,7 +377,7 @@ def leg_accuracy(sb, ctx, S, nthreads):
     return out
 
 
-def leg_c1(sb, ctx, nthreads, cpu_seconds=8.0):
+def leg_c1(sb, ctx, nthreads, cpu_seconds=12.0):
     """The CPU-runnable configuration: ER N=800 p=0.06 +-1 (J = -W), R=1,
     1000 bSB steps, dt=1.25, a0=1, c0 auto, keyed uniform init from seed 1.
     GPU: the thin R = 1 path (FP32), timed.  CPU: the oracle's FP32 loop (the

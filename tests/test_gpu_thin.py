@@ -33,7 +33,10 @@ def solve_forced(ctx, prob, family, shape, **kw):
 
 
 @pytest.mark.parametrize("variant", ["bsb", "asb", "dsb", "asb_m1"])
-@pytest.mark.parametrize("N,R", [(300, 1), (1100, 3), (2100, 4)])
+# R = 1, 2 and 3-4 run the kernels compiled for 1, 2 and 4 replicas; N covers
+# rows of one partial 16-element group (5), one group per lane (64, 300), two
+# (1025, 1100) and three (2100, 3000).
+@pytest.mark.parametrize("N,R", [(300, 1), (1100, 3), (2100, 4), (5, 2), (64, 2), (1025, 2), (3000, 1)])
 def test_thin_fp32_bit_exact_vs_lane_order_oracle(ctx, orc, variant, N, R):
     J = sb.generate(sb.GEN_SK if variant == "bsb" else sb.GEN_KN, N, 1.0, 21)
     v = sb.VARIANTS[variant]

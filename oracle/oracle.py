@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 ASB, BSB, DSB, ASB_M1 = 0, 1, 2, 3
-ORDER_SEQ, ORDER_LANES, ORDER_MFMA = 0, 1, 2  # field summation orders (sb_oracle.h)
+ORDER_SEQ, ORDER_LANES, ORDER_MFMA, ORDER_SPLIT = 0, 1, 2, 3  # field orders (sb_oracle.h)
 
 _lib = None
 
@@ -38,6 +38,8 @@ def load():
             "orc_field_f64": (None, [I, I, P, I, P, I, I, P]),
             "orc_field_f32_lanes": (None, [I, I, P, I, P, I, I, P]),
             "orc_field_f32_mfma": (None, [I, I, P, I, P, I, I, P]),
+            "orc_field_split": (None, [I, I, P, I, P, I, I, P]),
+            "orc_mfma_bf16": (F, [F, P, P]),
             "orc_bsb_update_f32": (None, [F, F, F, F, C.POINTER(F), C.POINTER(F)]),
             "orc_asb_local_f32": (None, [C.POINTER(F), C.POINTER(F), F, D, D, D, I]),
             "orc_m1_x_f32": (F, [F, F, D, D]),
@@ -122,9 +124,18 @@ def field_f32(J, x, use_sign=False, order=ORDER_SEQ):
     R, ld = x.shape
     F = np.zeros((R, ld), np.float32)
     fn = {ORDER_SEQ: load().orc_field_f32, ORDER_LANES: load().orc_field_f32_lanes,
-          ORDER_MFMA: load().orc_field_f32_mfma}[order]
+          ORDER_MFMA: load().orc_field_f32_mfma, ORDER_SPLIT: load().orc_field_split}[order]
     fn(N, R, _p(J), J.shape[1], _p(x), ld, int(use_sign), _p(F))
     return F
+
+
+def mfma_bf16(c, a, b):
+    """The oracle's model of one v_mfma_f32_32x32x16_bf16 output: c plus the
+    16 products of BF16 values a[k] b[k] (see orc_mfma_bf16)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == (16,) and b.shape == (16,)
+    return np.float32(load().orc_mfma_bf16(float(c), _p(a), _p(b)))
 
 
 def field_f64(J, x, use_sign=False):

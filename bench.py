@@ -222,7 +222,7 @@ def leg_batched(sb, ctx, n_inst, steps):
     return leg
 
 
-def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
+def leg_serving(sb, ctx, seconds, rate, drain_cap_s, chunk=50, sparse_density=0.0):
     rng = np.random.default_rng(5)
     n_req = int(rng.poisson(rate * seconds))
     arrivals = np.sort(rng.uniform(0.0, seconds, n_req))
@@ -231,7 +231,8 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
     dens = rng.uniform(0.01, 1.0, n_req)
     Rs = rng.choice([32, 64, 128, 256], n_req)
     steps = rng.integers(1000, 5001, n_req)
-    srv = sb.Server(ctx, hbm_budget_bytes=128 * GB, chunk_steps=50, precision="split")
+    srv = sb.Server(ctx, hbm_budget_bytes=128 * GB, chunk_steps=chunk, precision="split",
+                    sparse_density=sparse_density)
     tickets = [None] * n_req
     lat = {}
     done_evt = threading.Event()
@@ -296,7 +297,8 @@ def leg_serving(sb, ctx, seconds, rate, drain_cap_s):
     leg = {
         "workload": f"Poisson {rate} req/s for {seconds} s (seed 5): N log-uniform [100,8000], half diluted SK "
                     "(BF16 J) half ER +-1 (FP8 J), density U[0.01,1], R in {32,64,128,256}, steps U[1000,5000], "
-                    "bSB split precision, chunk 50 steps, HBM budget 128 GiB",
+                    f"bSB split precision, chunk {chunk} steps, HBM budget 128 GiB, sparse family at density "
+                    f"<= {sparse_density}",
         "requests": n_req, "completed": len(done_idx), "cancelled_at_drain_cap": n_req - len(done_idx),
         "p50_ms": float(np.percentile(L, 50)), "p99_ms": float(np.percentile(L, 99)),
         "mean_ms": float(L.mean()), "submit_window_s": submit_s, "total_s": total_s,
@@ -452,6 +454,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving leg")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-drain-cap", type=float, default=300.0, help="max seconds to drain after arrivals")
+    ap.add_argument("--serve-chunk", type=int, default=50, help="server steps between admission points")
+    ap.add_argument("--serve-sparse-density", type=float, default=0.0,
+                    help="requests at or below this density run the sparse family (0: never)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -523,7 +528,8 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = leg_batched(sb, ctx, args.batch_instances, args.batch_steps)
     if not args.no_serving:
-        out["serving"] = leg_serving(sb, ctx, args.serve_seconds, args.serve_rate, args.serve_drain_cap)
+        out["serving"] = leg_serving(sb, ctx, args.serve_seconds, args.serve_rate, args.serve_drain_cap,
+                                     args.serve_chunk, args.serve_sparse_density)
         out["p50_ms"] = out["serving"]["p50_ms"]
         out["p99_ms"] = out["serving"]["p99_ms"]
     threads = min(16, os.cpu_count() or 1)

@@ -139,3 +139,17 @@ def test_cli_usage_exits_zero():
     exe = os.path.join(ROOT, "annealing_amd", "bin", "sb_solve")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "Usage" in r.stdout
+
+
+def test_missing_library_raises():
+    # No CPU fallback: without the built HIP library the binding refuses to load.
+    code = "from annealing_amd import sb; sb.load('/nonexistent/libsbamd.so')"
+    r = subprocess.run(["python3", "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "not built" in r.stderr
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_context_without_gpu_is_an_error():
+    # Without a GPU the solver reports an error status; it never computes on the CPU.
+    with pytest.raises(sb.SBError):
+        sb.Context(0)

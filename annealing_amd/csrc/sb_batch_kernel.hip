@@ -128,6 +128,10 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         const int sel = (I.family * 2 + (I.variant == SB_DSB)) * 8 + I.bshape;
         unsigned short *sm16 = (unsigned short *)smem_dyn;
         // One case per (family, sign B, batch shape); must follow the SB_BATCH_* lists.
+        // Thin items load one 16-element group at a time (no second group in
+        // flight): at two waves per SIMD the kernel has 256 VGPRs, and the
+        // pipelined thin loop at four replicas would spill the whole kernel
+        // to scratch.  The sums are the same either way.
         switch (sel) {
         case (FAM_FP32 * 2 + 0) * 8 + 0:
         case (FAM_FP32 * 2 + 1) * 8 + 0: fp32_tile<2, 2, 1, 1>(a, item.tile, (float *)smem_dyn); break;
@@ -152,12 +156,12 @@ __global__ __launch_bounds__(256, 2) void k_batch_step(const BatchInstance *__re
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 1: dsb_fp8_tile<2, 2, 2, 1, true>(a, item.tile, smem_dyn); break;
         case (FAM_DSB_FP8 * 2 + 1) * 8 + 2: dsb_fp8_tile<4, 1, 1, 1, true>(a, item.tile, smem_dyn); break;
 #define SB_THIN_CASES(fam, jb)                                                                          \
-        case (fam * 2 + 0) * 8 + 0: thin_tile<1, jb, false, kThinMaxR>(a, item.tile); break;            \
-        case (fam * 2 + 0) * 8 + 1: thin_tile<2, jb, false, kThinMaxR>(a, item.tile); break;            \
-        case (fam * 2 + 0) * 8 + 2: thin_tile<4, jb, false, kThinMaxR>(a, item.tile); break;            \
-        case (fam * 2 + 1) * 8 + 0: thin_tile<1, jb, true, kThinMaxR>(a, item.tile); break;             \
-        case (fam * 2 + 1) * 8 + 1: thin_tile<2, jb, true, kThinMaxR>(a, item.tile); break;             \
-        case (fam * 2 + 1) * 8 + 2: thin_tile<4, jb, true, kThinMaxR>(a, item.tile); break;
+        case (fam * 2 + 0) * 8 + 0: thin_tile<1, jb, false, kThinMaxR, 0>(a, item.tile); break;         \
+        case (fam * 2 + 0) * 8 + 1: thin_tile<2, jb, false, kThinMaxR, 0>(a, item.tile); break;         \
+        case (fam * 2 + 0) * 8 + 2: thin_tile<4, jb, false, kThinMaxR, 0>(a, item.tile); break;         \
+        case (fam * 2 + 1) * 8 + 0: thin_tile<1, jb, true, kThinMaxR, 0>(a, item.tile); break;          \
+        case (fam * 2 + 1) * 8 + 1: thin_tile<2, jb, true, kThinMaxR, 0>(a, item.tile); break;          \
+        case (fam * 2 + 1) * 8 + 2: thin_tile<4, jb, true, kThinMaxR, 0>(a, item.tile); break;
         SB_THIN_CASES(FAM_THIN_F32, 4)
         SB_THIN_CASES(FAM_THIN_BF16, 2)
         SB_THIN_CASES(FAM_THIN_FP8, 1)

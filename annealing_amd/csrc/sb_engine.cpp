@@ -283,22 +283,30 @@ int Engine::rebuild()
     rc = ensure_tables(std::max<size_t>(n, 1), std::max<size_t>(total_items, 1));
     if (rc) return rc;
 
-    // Instances with a small J keep all their tiles on one XCD (largest first
-    // onto the least loaded XCD), so J and state stay in that XCD's L2.  An
-    // instance whose J is larger than that is split over the XCDs in bands of
-    // row tiles (XCD x streams band x of J's rows): a few large instances then
-    // still occupy the whole device.  Each XCD's tiles go, largest first, to
-    // its least loaded workgroup (workgroup w runs on XCD w % 8).
+    // Instances keep all their tiles on one XCD (largest first onto the least
+    // loaded XCD), so J and state stay in that XCD's L2.  An instance too
+    // large for that (more than 1/16 of the step's work, and a J beyond 2 MB)
+    // is split over the XCDs in bands of row tiles (XCD x streams band x of
+    // J's rows): a few large instances then still occupy the whole device.
+    // Splitting costs state traffic (every XCD reads the instance's whole X),
+    // so many mid-sized instances stay whole: for 512 instances of N up to
+    // 4096 at R = 64, splitting every J beyond 2 MB took 1255-1277 us per
+    // batched step in the split form and 3529-3590 us in FP32, keeping them
+    // whole 1139 and 3104 us.
+    // Each XCD's tiles go, largest first, to its least loaded workgroup
+    // (workgroup w runs on XCD w % 8).
     const int nx = 8;
     std::vector<int> order(n);
     for (size_t i = 0; i < n; ++i) order[i] = (int)i;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return slot_cost[a] > slot_cost[b]; });
+    double total_cost = 0.0;
+    for (size_t i = 0; i < n; ++i) total_cost += slot_cost[i];
     auto spread = [&](int i) {
         const BatchInstance &I = h_inst_[i];
         const double bj = I.family == FAM_FP32 || I.family == FAM_THIN_F32 ? 4.0
                           : I.family == FAM_SPLIT_BF16 || I.family == FAM_THIN_BF16 ? 2.0 : 1.0;
         const double jbytes = I.family == FAM_SPARSE ? 8.0 * (double)slots_[i].prob->nnz : (double)I.N * I.ldx * bj;
-        return I.tiles_m >= nx && jbytes > 2.0 * (1 << 20);
+        return I.tiles_m >= nx && jbytes > 2.0 * (1 << 20) && slot_cost[i] > total_cost / (2 * nx);
     };
     std::vector<double> xcd_load(nx, 0.0);
     std::vector<std::vector<Job>> xcd_jobs(nx);

@@ -852,7 +852,7 @@ struct ThinGroup {
 // so a row of up to 2048 spins takes one memory round trip and a longer row
 // streams with a group in flight.  A lane sums a group only if it starts below
 // N, exactly as above.
-template <int ROWS, int JB, bool SGN, int RM>
+template <int ROWS, int JB, bool SGN, int RM, int VBUDGET = 192>
 __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
 {
     // wave (hence row0 and the buffer resources) is uniform across the wave.
@@ -917,7 +917,10 @@ __device__ __forceinline__ void thin_tile(const StepArgs &a, int tile)
                 for (int e = 0; e < 16; ++e) acc[q][r] = __builtin_fmaf(jv[e], xv[r][e], acc[q][r]);
         }
     };
-    constexpr bool PIPE = 2 * (ROWS * 4 * JB + RM * 16) <= 192;
+    // Two groups in flight when their load registers fit VBUDGET (the
+    // single-instance kernels have the whole register file; the batched
+    // kernel, at two waves per SIMD, passes 0).
+    constexpr bool PIPE = 2 * (ROWS * 4 * JB + RM * 16) <= VBUDGET;
     const int groups = (a.N + 1023) >> 10;  // wave-uniform
     int k0 = 16 * lane;
     if constexpr (PIPE) {

@@ -153,3 +153,39 @@ def test_context_without_gpu_is_an_error():
     # Without a GPU the solver reports an error status; it never computes on the CPU.
     with pytest.raises(sb.SBError):
         sb.Context(0)
+
+
+def kernel_scratch(text):
+    """{kernel: scratch bytes per lane} from hipcc's kernel-resource-usage remarks."""
+    out, name = {}, None
+    for line in text.splitlines():
+        m = re.search(r"remark: Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"remark:\s+ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            out[name] = int(m.group(1))
+    return out
+
+
+def test_scratch_parser_finds_a_spill():
+    text = ("a.hip:1:1: remark: Function Name: k_one [-Rpass-analysis=kernel-resource-usage]\n"
+            "a.hip:1:1: remark:     ScratchSize [bytes/lane]: 0 [-Rpass-analysis=kernel-resource-usage]\n"
+            "a.hip:9:1: remark: Function Name: k_two [-Rpass-analysis=kernel-resource-usage]\n"
+            "a.hip:9:1: remark:     ScratchSize [bytes/lane]: 264 [-Rpass-analysis=kernel-resource-usage]\n")
+    assert kernel_scratch(text) == {"k_one": 0, "k_two": 264}
+
+
+def test_no_kernel_spills_to_scratch():
+    # The Makefile keeps hipcc's resource-usage remarks per translation unit.
+    # A spill to scratch (e.g. the batched kernel, at two waves per SIMD and
+    # 256 VGPRs, holding a tile that needs more) slows every launch.
+    import glob
+    files = sorted(glob.glob(os.path.join(ROOT, "annealing_amd", "_build", "*.usage")))
+    if not files:
+        pytest.skip("kernels not built by annealing_amd/csrc/Makefile in this tree")
+    usage = {}
+    for f in files:
+        usage.update(kernel_scratch(open(f).read()))
+    assert len(usage) >= 150 and any("k_batch_step" in k for k in usage)
+    assert {k: v for k, v in usage.items() if v} == {}

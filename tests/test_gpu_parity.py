@@ -264,6 +264,29 @@ def test_cli_writes_output_txt(ctx, orc, mode):
     assert np.array_equal(H, orc.energy_i64(J, so))
 
 
+def test_cli_split_matches_oracle_split_order(ctx, orc):
+    # --precision split through the CLI: integer J stored as FP8 (exact), X as
+    # BF16 hi + lo on the BF16 MFMA; the oracle's model of that MFMA gives the
+    # same spins, hence the same int64 energies.
+    N, R = 40, 6
+    J = kn_pm1(N, 79)
+    lines = [f"{N}"] + [f"{i} {j} {int(J[i, j])}" for i in range(N) for j in range(i + 1, N)]
+    exe = os.path.join(ROOT, "annealing_amd", "bin", "sb_solve")
+    with tempfile.TemporaryDirectory() as d:
+        inst = os.path.join(d, "k40.txt")
+        open(inst, "w").write("\n".join(lines) + "\n")
+        r = subprocess.run([exe, inst, "--variant", "bsb", "--precision", "split", "--steps", "200",
+                            "--replicas", str(R), "--init", "uniform", "--seed", "3", "--dt", "0.5",
+                            "--spins", "spins.txt"], cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        H = np.loadtxt(os.path.join(d, "output.txt"), dtype=np.int64)
+        S = np.loadtxt(os.path.join(d, "spins.txt"), dtype=np.int8)
+    xo, _, _ = orc.solve_f32(orc.BSB, J, R, 200, init="uniform", seed=3, dt=0.5, order=orc.ORDER_SPLIT)
+    so = orc.spins(np.ascontiguousarray(xo[:, :N]), N)
+    assert np.array_equal(S, so)
+    assert np.array_equal(H, orc.energy_i64(J, so))
+
+
 def test_cli_fractional_field_writes_fp64_energies(ctx, orc):
     # Integer J with a fractional field line: energies are not integers, so
     # output.txt carries the FP64 values (not truncated integers).

@@ -177,6 +177,14 @@ def test_c5_server_size_extremes(ctx, orc):
             x, y = orc.init_state("uniform", N, R, seed=gseed)
             orc.run_f32(orc.DSB, J, x, y, 0, steps, steps, 1.0, 1.0, r.c0, nthreads=THREADS)
             assert np.array_equal(r.spins, orc.spins(x, N)), i
+        else:
+            # bSB in the split form: two sampled replicas follow the oracle's
+            # model of the BF16 MFMA bit for bit (a replica depends only on itself).
+            x0, y0 = orc.init_state("uniform", N, R, seed=gseed)
+            pk = [0, R - 1]
+            x, y = np.ascontiguousarray(x0[pk]), np.ascontiguousarray(y0[pk])
+            orc.run_f32(orc.BSB, J, x, y, 0, steps, steps, 1.0, 1.0, r.c0, nthreads=THREADS, order=orc.ORDER_SPLIT)
+            assert np.array_equal(r.spins[pk], orc.spins(x, N)), i
 
 
 def test_server_thin_fp32_requests_equal_lane_oracle(ctx, orc):

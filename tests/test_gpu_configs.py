@@ -122,11 +122,20 @@ def test_c4_batched_512_instances(orc):
             assert s.exact
             assert np.array_equal(s.energy_i64, orc.energy_i64(J, s.spins)), i
             assert np.array_equal(s.cut, brute_cut(J, s.spins)), i
+            N = int(Ns[i])
             if variants[i] == "dsb":
-                x, y = orc.init_state("uniform", int(Ns[i]), 8, seed=i)
+                x, y = orc.init_state("uniform", N, 8, seed=i)
                 orc.run_f32(orc.DSB, J, x, y, 0, steps, steps, 1.0, 1.0, s.c0, nthreads=THREADS)
-                N = int(Ns[i])
                 assert np.array_equal(s.x[:8], x[:, :N]) and np.array_equal(s.y[:8], y[:, :N]), i
+            else:
+                # bSB split instances follow the oracle's BF16-MFMA model bit
+                # for bit (two sampled replicas).
+                x0, y0 = orc.init_state("uniform", N, R, seed=i)
+                pk = [0, R - 1]
+                x, y = np.ascontiguousarray(x0[pk]), np.ascontiguousarray(y0[pk])
+                orc.run_f32(orc.BSB, J, x, y, 0, steps, steps, 1.0, 1.0, s.c0, nthreads=THREADS,
+                            order=orc.ORDER_SPLIT)
+                assert np.array_equal(s.x[pk], x[:, :N]) and np.array_equal(s.y[pk], y[:, :N]), i
         for p in probs:
             p.close()
 

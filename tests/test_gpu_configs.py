@@ -196,9 +196,10 @@ def test_c5_server_size_extremes(ctx, orc):
             assert np.array_equal(r.spins[pk], orc.spins(x, N)), i
 
 
-def test_server_thin_fp32_requests_equal_lane_oracle(ctx, orc):
-    # R <= 4 requests run the thin kernels inside the batched launch; their
-    # FP32 trajectory is the oracle's 64-lane order bit for bit, in any mix.
+def test_server_fp32_requests_equal_oracle(ctx, orc):
+    # R <= 4 requests run the thin kernels inside the batched launch, larger R
+    # the FP32 MFMA tiles; their trajectories are the oracle's 64-lane and MFMA
+    # orders bit for bit, in any mix.
     specs = [(sb.GEN_ER, 800, 0.06, 21, "bsb", 1, 300), (sb.GEN_SK, 1500, 1.0, 22, "bsb", 3, 200),
              (sb.GEN_ER, 600, 0.3, 23, "bsb", 64, 150), (sb.GEN_KN, 333, 1.0, 24, "dsb", 2, 250)]
     with sb.Server(ctx, 16 * GB, chunk_steps=20, precision="fp32") as srv:
@@ -206,12 +207,11 @@ def test_server_thin_fp32_requests_equal_lane_oracle(ctx, orc):
                                         init="uniform", seed=g, dt=1.0) for k, N, p, g, v, R, s in specs]
         res = [srv.wait(t) for t in tickets]
     for (k, N, p, g, v, R, s), r in zip(specs, res):
-        if R > 4:
-            continue
         prob = ctx.generate(k, N, p, g)
         J = prob.read_j()
         prob.close()
         x, y = orc.init_state("uniform", N, R, seed=g)
         var = orc.BSB if v == "bsb" else orc.DSB
-        orc.run_f32(var, J, x, y, 0, s, s, 1.0, 1.0, r.c0, order=orc.ORDER_LANES)
+        order = orc.ORDER_LANES if R <= 4 else orc.ORDER_MFMA
+        orc.run_f32(var, J, x, y, 0, s, s, 1.0, 1.0, r.c0, nthreads=THREADS, order=order)
         assert np.array_equal(r.spins, orc.spins(x, N)), (N, R)

@@ -22,6 +22,8 @@ static void usage()
         "  --variant asb|asb-m1|bsb|dsb  update rule (default asb, m2_opt.cu; asb-m1 is\n"
         "                            Bifurcation_m1.cu.c)\n"
         "  --precision fp32|split|ordered\n"
+        "                            split (alias fp8bf16): J as FP8 E4M3 when integer with\n"
+        "                            |J| <= 16, else BF16; X as BF16 hi + lo; FP32 sums\n"
         "  --steps N                 (default 100)\n"
         "  --dt X                    coupling step DeltaT (default 1.0)\n"
         "  --a0 X                    detune / a0 (default 1)\n"
@@ -92,7 +94,7 @@ int main(int argc, char **argv)
         } else if (a == "--precision") {
             std::string v = need("--precision");
             if (v == "fp32") p.precision = SB_PREC_FP32;
-            else if (v == "split") p.precision = SB_PREC_SPLIT;
+            else if (v == "split" || v == "fp8bf16") p.precision = SB_PREC_SPLIT;
             else if (v == "ordered") p.precision = SB_PREC_ORDERED;
             else { std::fprintf(stderr, "bad --precision %s\n", v.c_str()); return 2; }
         } else if (a == "--init") {

@@ -264,7 +264,8 @@ def test_cli_writes_output_txt(ctx, orc, mode):
     assert np.array_equal(H, orc.energy_i64(J, so))
 
 
-def test_cli_split_matches_oracle_split_order(ctx, orc):
+@pytest.mark.parametrize("name", ["split", "fp8bf16"])
+def test_cli_split_matches_oracle_split_order(ctx, orc, name):
     # --precision split through the CLI: integer J stored as FP8 (exact), X as
     # BF16 hi + lo on the BF16 MFMA; the oracle's model of that MFMA gives the
     # same spins, hence the same int64 energies.
@@ -275,7 +276,7 @@ def test_cli_split_matches_oracle_split_order(ctx, orc):
     with tempfile.TemporaryDirectory() as d:
         inst = os.path.join(d, "k40.txt")
         open(inst, "w").write("\n".join(lines) + "\n")
-        r = subprocess.run([exe, inst, "--variant", "bsb", "--precision", "split", "--steps", "200",
+        r = subprocess.run([exe, inst, "--variant", "bsb", "--precision", name, "--steps", "200",
                             "--replicas", str(R), "--init", "uniform", "--seed", "3", "--dt", "0.5",
                             "--spins", "spins.txt"], cwd=d, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr

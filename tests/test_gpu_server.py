@@ -81,6 +81,18 @@ def test_budget_rejection_and_backfill():
         assert 1 <= st.max_active_seen <= 2
 
 
+def test_invalid_server_options_are_status_codes():
+    with sb.Context(0) as ctx:
+        for kw in ({"hbm_budget_bytes": 0}, {"hbm_budget_bytes": GB, "chunk_steps": 0},
+                   {"hbm_budget_bytes": GB, "sparse_density": -0.1}, {"hbm_budget_bytes": GB, "sparse_density": 1.5},
+                   {"hbm_budget_bytes": GB, "sparse_density": float("nan")}):
+            with pytest.raises(sb.SBError) as e:
+                sb.Server(ctx, **kw)
+            assert e.value.status == 1, kw
+        with sb.Server(ctx, GB, sparse_density=1.0):
+            pass
+
+
 def test_cancel_and_timeout():
     with sb.Context(0) as ctx, sb.Server(ctx, 8 * GB, chunk_steps=5) as srv:
         long_t = srv.submit_generated(sb.GEN_KN, 2000, 1.0, 1, variant="bsb", precision="split", steps=200000,

@@ -46,6 +46,20 @@ def test_split_tiles_bit_exact_vs_oracle(ctx, orc, variant, kind, N, R, steps):
     prob.close()
 
 
+@pytest.mark.parametrize("N", [1, 2, 3, 17, 63])
+@pytest.mark.parametrize("R", [5, 40])
+def test_split_tiny_instances_bit_exact(ctx, orc, N, R):
+    # Instances smaller than one k16 group or one 32-spin block: the padding
+    # (zero J columns and rows, zero state) contributes nothing.
+    J = sb.generate(sb.GEN_SK, N, 1.0, 100 + N) if N > 1 else np.zeros((1, 1), np.float32)
+    prob = ctx.problem(J)
+    sol = ctx.solve(prob, state=True, variant="bsb", precision="split", steps=30, replicas=R, init="uniform",
+                    seed=N, dt=0.5, c0=0.3)
+    xo, yo, _ = orc.solve_f32(orc.BSB, J, R, 30, init="uniform", seed=N, dt=0.5, c0=0.3, order=orc.ORDER_SPLIT)
+    assert np.array_equal(sol.x, xo[:, :N]) and np.array_equal(sol.y, yo[:, :N])
+    prob.close()
+
+
 def test_split_field_operator_bit_exact(ctx, orc):
     # sb_field through the split tiles against orc_field_split, on state with
     # a wide spread of magnitudes (exercises the truncation windows).

@@ -54,6 +54,21 @@ def test_thin_fp32_bit_exact_vs_lane_order_oracle(ctx, orc, variant, N, R):
 
 
 @pytest.mark.parametrize("variant", ["bsb", "dsb"])
+@pytest.mark.parametrize("N", [1, 2, 3, 16, 17])
+@pytest.mark.parametrize("R", [1, 2, 4])
+def test_thin_tiny_instances_bit_exact(ctx, orc, variant, N, R):
+    # Fewer spins than one 16-element group, and one wave's rows mostly past N.
+    J = sb.generate(sb.GEN_KN, N, 1.0, 30 + N) if N > 1 else np.zeros((1, 1), np.float32)
+    v = sb.VARIANTS[variant]
+    prob = ctx.problem(J)
+    sol = ctx.solve(prob, state=True, variant=v, precision="fp32", steps=25, replicas=R, init="uniform", seed=N,
+                    dt=0.5, c0=0.3)
+    xo, yo, _ = orc.solve_f32(v, J, R, 25, init="uniform", seed=N, dt=0.5, c0=0.3, order=orc.ORDER_LANES)
+    assert np.array_equal(sol.x, xo[:, :N]) and np.array_equal(sol.y, yo[:, :N])
+    prob.close()
+
+
+@pytest.mark.parametrize("variant", ["bsb", "dsb"])
 def test_thin_every_shape_and_j_format_identical(ctx, variant):
     # Integer J: FP32, BF16 and FP8 hold the same values, so every thin family
     # and shape must give the same trajectory.

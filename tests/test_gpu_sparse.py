@@ -43,6 +43,27 @@ def test_sparse_bit_exact_vs_sequential_oracle(ctx, orc, variant, N, R):
     prob.close()
 
 
+@pytest.mark.parametrize("N", [1, 2, 5, 40])
+@pytest.mark.parametrize("R", [1, 64, 70])
+def test_sparse_empty_rows_and_tiny_instances(ctx, orc, N, R):
+    # Spins without a nonzero (every third spin isolated), an all-zero J
+    # (N <= 2: no nonzeros at all) and fewer spins than a workgroup's four.
+    J = np.zeros((N, N), np.float32)
+    if N >= 5:
+        J = sb.generate(sb.GEN_ER, N, 0.3, 90 + N)
+        J[:, ::3] = 0.0
+        J[::3, :] = 0.0
+    prob = sparse_problem(ctx, J)
+    assert prob.nnz == int(np.count_nonzero(J))
+    for variant in ("bsb", "dsb"):
+        v = sb.VARIANTS[variant]
+        sol = ctx.solve(prob, state=True, variant=v, precision="fp32", steps=30, replicas=R, init="uniform",
+                        seed=N, dt=0.5, c0=0.3)
+        xo, yo, _ = orc.solve_f32(v, J, R, 30, init="uniform", seed=N, dt=0.5, c0=0.3)
+        assert np.array_equal(sol.x, xo[:, :N]) and np.array_equal(sol.y, yo[:, :N]), variant
+    prob.close()
+
+
 def test_sparse_with_field_and_energies(ctx, orc):
     N, R, steps = 640, 24, 50
     J = sb.generate(sb.GEN_ER, N, 0.03, 81)

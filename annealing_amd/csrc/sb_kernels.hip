@@ -11,27 +11,33 @@
 
 namespace sbk {
 
+// Kernels indexed by replica in blockIdx.y stride over replicas, so R is not
+// bounded by the 65535 limit of a grid's y dimension.
+constexpr int kMaxGridY = 65535;
+static inline unsigned grid_y(int R) { return (unsigned)(R < kMaxGridY ? R : kMaxGridY); }
+
 // Ordered-sum debug kernel: one thread per (spin, replica), F_i summed over
 // j = 0..N-1 with fmaf in order (Bifurcation_m2.cu.c:93-97).
 __global__ __launch_bounds__(256) void k_step_ordered(StepArgs a)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (i >= a.N || r >= a.R) return;
+    if (i >= a.N) return;
     const float *Jr = (const float *)a.J + (size_t)i * a.ldj;
-    const float *xr = a.X + (size_t)r * a.ldx;
     const bool sgn = a.c.variant == SB_DSB;
-    float F = 0.0f;
-    for (int j = 0; j < a.N; ++j) {
-        float v = xr[j];
-        if (sgn) v = sgn1(v);
-        F = __builtin_fmaf(Jr[j], v, F);
+    for (int r = blockIdx.y; r < a.R; r += gridDim.y) {
+        const float *xr = a.X + (size_t)r * a.ldx;
+        float F = 0.0f;
+        for (int j = 0; j < a.N; ++j) {
+            float v = xr[j];
+            if (sgn) v = sgn1(v);
+            F = __builtin_fmaf(Jr[j], v, F);
+        }
+        if (a.h) F = F + a.h[i];  // external field, after the sum
+        float x = xr[i], y = a.Y[(size_t)r * a.ldx + i];
+        sb_update(a.c, F, x, y);
+        a.Xn[(size_t)r * a.ldx + i] = x;
+        a.Y[(size_t)r * a.ldx + i] = y;
     }
-    if (a.h) F = F + a.h[i];  // external field, after the sum
-    float x = xr[i], y = a.Y[(size_t)r * a.ldx + i];
-    sb_update(a.c, F, x, y);
-    a.Xn[(size_t)r * a.ldx + i] = x;
-    a.Y[(size_t)r * a.ldx + i] = y;
 }
 
 #define SB_SHAPE_ENTRY(wm, wn, bm, bn) TileShape{wm, wn, bm, bn, 0},
@@ -167,7 +173,7 @@ hipError_t launch_to_replica_major(const float *XT, int ldr, int N, int R, float
 hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
 {
     if (a.N <= 0 || a.R <= 0) return hipSuccess;
-    dim3 grid((a.N + 255) / 256, a.R);
+    dim3 grid((a.N + 255) / 256, grid_y(a.R));
     hipLaunchKernelGGL(k_step_ordered, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -177,21 +183,21 @@ hipError_t launch_step_ordered(const StepArgs &a, hipStream_t s)
 
 __global__ __launch_bounds__(256) void k_generate(int kind, int N, double p, uint64_t seed, float *J, int ld)
 {
-    const int i = blockIdx.y;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ld; j += gridDim.x * blockDim.x) {
-        float v = 0.0f;
-        if (j < N && j != i)
-            v = i < j ? gen_coupling(kind, N, p, seed, (uint32_t)i, (uint32_t)j)
-                      : gen_coupling(kind, N, p, seed, (uint32_t)j, (uint32_t)i);
-        J[(size_t)i * ld + j] = v;
-    }
+    for (int i = blockIdx.y; i < N; i += gridDim.y)
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ld; j += gridDim.x * blockDim.x) {
+            float v = 0.0f;
+            if (j < N && j != i)
+                v = i < j ? gen_coupling(kind, N, p, seed, (uint32_t)i, (uint32_t)j)
+                          : gen_coupling(kind, N, p, seed, (uint32_t)j, (uint32_t)i);
+            J[(size_t)i * ld + j] = v;
+        }
 }
 
 hipError_t launch_generate(int kind, int N, double p, uint64_t seed, float *J, int ld, hipStream_t s)
 {
     if (N <= 0) return hipSuccess;
     int gx = (ld + 255) / 256;
-    hipLaunchKernelGGL(k_generate, dim3(gx < 16 ? gx : 16, N), dim3(256), 0, s, kind, N, p, seed, J, ld);
+    hipLaunchKernelGGL(k_generate, dim3(gx < 16 ? gx : 16, grid_y(N)), dim3(256), 0, s, kind, N, p, seed, J, ld);
     return hipGetLastError();
 }
 
@@ -304,17 +310,18 @@ __global__ __launch_bounds__(256) void k_init_uniform(uint64_t seed, int N, int 
                                                       float *Y, float *X2)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (i >= ldx || r >= R) return;
-    size_t o = (size_t)r * ldx + i;
-    float x = 0.0f, y = 0.0f;
-    if (i < N) {
-        x = keyed_value(keyed_u24(seed, (uint32_t)r, (uint32_t)i, 0u));
-        y = keyed_value(keyed_u24(seed, (uint32_t)r, (uint32_t)i, 1u));
+    if (i >= ldx) return;
+    for (int r = blockIdx.y; r < R; r += gridDim.y) {
+        size_t o = (size_t)r * ldx + i;
+        float x = 0.0f, y = 0.0f;
+        if (i < N) {
+            x = keyed_value(keyed_u24(seed, (uint32_t)r, (uint32_t)i, 0u));
+            y = keyed_value(keyed_u24(seed, (uint32_t)r, (uint32_t)i, 1u));
+        }
+        X[o] = x;
+        Y[o] = y;
+        if (X2) X2[o] = 0.0f;
     }
-    X[o] = x;
-    Y[o] = y;
-    if (X2) X2[o] = 0.0f;
 }
 
 // One thread per spin walks its xorshift32 stream over the replicas, as the
@@ -346,8 +353,8 @@ hipError_t launch_init(int kind, uint64_t seed, int N, int R, int ldx, float *X,
         hipLaunchKernelGGL(k_init_reference, dim3((ldx + 255) / 256), dim3(256), 0, s, (uint32_t)seed, N, R,
                            ldx, X, Y, X2);
     } else {
-        hipLaunchKernelGGL(k_init_uniform, dim3((ldx + 255) / 256, R), dim3(256), 0, s, seed, N, R, ldx, X,
-                           Y, X2);
+        hipLaunchKernelGGL(k_init_uniform, dim3((ldx + 255) / 256, grid_y(R)), dim3(256), 0, s, seed, N, R, ldx,
+                           X, Y, X2);
     }
     return hipGetLastError();
 }
@@ -356,23 +363,24 @@ __global__ __launch_bounds__(256) void k_asb_local(sb_step_coeffs c, int N, int 
                                                    float *Y)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (i >= N || r >= R) return;
-    size_t o = (size_t)r * ldx + i;
-    float x = X[o], y = Y[o];
-    if (c.variant == SB_ASB_M1)
-        x = m1_x(x, y, c.a0, c.dts);  // step 0's x-step
-    else
-        asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
-    X[o] = x;
-    Y[o] = y;
+    if (i >= N) return;
+    for (int r = blockIdx.y; r < R; r += gridDim.y) {
+        size_t o = (size_t)r * ldx + i;
+        float x = X[o], y = Y[o];
+        if (c.variant == SB_ASB_M1)
+            x = m1_x(x, y, c.a0, c.dts);  // step 0's x-step
+        else
+            asb_local(x, y, c.p_next, c.a0, c.K, c.dts, c.M);
+        X[o] = x;
+        Y[o] = y;
+    }
 }
 
 hipError_t launch_asb_local(const sb_step_coeffs &c, int N, int R, int ldx, float *X, float *Y,
                             hipStream_t s)
 {
     if (N <= 0 || R <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_asb_local, dim3((N + 255) / 256, R), dim3(256), 0, s, c, N, R, ldx, X, Y);
+    hipLaunchKernelGGL(k_asb_local, dim3((N + 255) / 256, grid_y(R)), dim3(256), 0, s, c, N, R, ldx, X, Y);
     return hipGetLastError();
 }
 
@@ -708,30 +716,29 @@ hipError_t noalloc_scratch_address(int device, double **ptr)
 __global__ __launch_bounds__(256) void k_spins(const float *X, int ldx, int N, int R, int8_t *S)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (i >= N || r >= R) return;
-    S[(size_t)r * N + i] = X[(size_t)r * ldx + i] > 0.0f ? (int8_t)1 : (int8_t)-1;
+    if (i >= N) return;
+    for (int r = blockIdx.y; r < R; r += gridDim.y)
+        S[(size_t)r * N + i] = X[(size_t)r * ldx + i] > 0.0f ? (int8_t)1 : (int8_t)-1;
 }
 
 hipError_t launch_spins(const float *X, int ldx, int N, int R, int8_t *S, hipStream_t s)
 {
     if (N <= 0 || R <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_spins, dim3((N + 255) / 256, R), dim3(256), 0, s, X, ldx, N, R, S);
+    hipLaunchKernelGGL(k_spins, dim3((N + 255) / 256, grid_y(R)), dim3(256), 0, s, X, ldx, N, R, S);
     return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void k_compact(const float *X, int ldx, int N, int R, float *out)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = blockIdx.y;
-    if (i >= N || r >= R) return;
-    out[(size_t)r * N + i] = X[(size_t)r * ldx + i];
+    if (i >= N) return;
+    for (int r = blockIdx.y; r < R; r += gridDim.y) out[(size_t)r * N + i] = X[(size_t)r * ldx + i];
 }
 
 hipError_t launch_compact(const float *X, int ldx, int N, int R, float *out, hipStream_t s)
 {
     if (N <= 0 || R <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_compact, dim3((N + 255) / 256, R), dim3(256), 0, s, X, ldx, N, R, out);
+    hipLaunchKernelGGL(k_compact, dim3((N + 255) / 256, grid_y(R)), dim3(256), 0, s, X, ldx, N, R, out);
     return hipGetLastError();
 }
 

@@ -234,6 +234,24 @@ def test_energy_kernel_exact(ctx, orc):
     assert np.allclose(sol.energy, orc.energy_f64(J, sol.spins), rtol=1e-12, atol=1e-10)
 
 
+def test_replicas_beyond_the_grid_y_limit(ctx, orc):
+    # R above 65535, the y extent of a launch grid: the init, aSB prologue,
+    # ordered step and readout kernels stride over replicas.
+    N, R, steps = 20, 70000, 5
+    J = kn_pm1(N, 5)
+    prob = ctx.problem(J)
+    for variant, precision, order in (("asb", "ordered", orc.ORDER_SEQ), ("bsb", "fp32", orc.ORDER_MFMA)):
+        v = sb.VARIANTS[variant]
+        sol = ctx.solve(prob, state=True, variant=v, precision=precision, steps=steps, replicas=R, init="uniform",
+                        seed=2, dt=0.5)
+        xo, yo, _ = orc.solve_f32(v, J, R, steps, init="uniform", seed=2, dt=0.5, order=order)
+        assert np.array_equal(sol.x, xo[:, :N]) and np.array_equal(sol.y, yo[:, :N]), variant
+        so = orc.spins(np.ascontiguousarray(xo[:, :N]), N)
+        assert np.array_equal(sol.spins, so), variant
+        assert np.array_equal(sol.energy_i64, orc.energy_i64(J, so)), variant
+    prob.close()
+
+
 # ---- errors and the CLI boundary --------------------------------------------------
 
 def test_invalid_params_rejected(ctx):

@@ -53,3 +53,28 @@ def test_concurrent_solves_on_one_context():
             serial = run_jobs(ctx, jobs, 100 * (t + 1))
             for k, (a, b) in enumerate(zip(results[t], serial)):
                 assert all(np.array_equal(u, v) for u, v in zip(a, b)), (t, k)
+
+
+def test_server_and_direct_solves_share_a_context():
+    # A server's worker thread and direct solves from this thread interleave on
+    # one context; each returns what it returns alone.
+    reqs = [(sb.GEN_ER, 600, 0.1, 31, "bsb", 64, 120), (sb.GEN_SK_DILUTED, 900, 0.5, 32, "bsb", 32, 90),
+            (sb.GEN_KN, 300, 1.0, 33, "dsb", 128, 150), (sb.GEN_ER, 1500, 0.02, 34, "bsb", 8, 60)]
+
+    def submit(srv, spec):
+        kind, N, p, g, v, R, steps = spec
+        return srv.submit_generated(kind, N, p, g, variant=v, precision="split", steps=steps, replicas=R,
+                                    init="uniform", seed=g, dt=0.5)
+
+    with sb.Context(0) as ctx:
+        with sb.Server(ctx, 16 * (1 << 30), chunk_steps=10, precision="split") as srv:
+            tickets = [submit(srv, s) for s in reqs]
+            direct = run_jobs(ctx, JOBS[1] + JOBS[3], 500)
+            mixed = [srv.wait(t, timeout_ms=120000) for t in tickets]
+        with sb.Server(ctx, 16 * (1 << 30), chunk_steps=10, precision="split") as srv:
+            alone = [srv.wait(submit(srv, s), timeout_ms=120000) for s in reqs]
+        serial = run_jobs(ctx, JOBS[1] + JOBS[3], 500)
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a.spins, m.spins) and np.array_equal(a.energy, m.energy), i
+    for k, (a, b) in enumerate(zip(direct, serial)):
+        assert all(np.array_equal(u, v) for u, v in zip(a, b)), k

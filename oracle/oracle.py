@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# SB_ORACLE_LIB selects another build of the same source (the sanitized one of
+# `make -C oracle sanitize`).
+LIB_PATH = os.environ.get("SB_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 ASB, BSB, DSB, ASB_M1 = 0, 1, 2, 3
 ORDER_SEQ, ORDER_LANES, ORDER_MFMA, ORDER_SPLIT = 0, 1, 2, 3  # field orders (sb_oracle.h)

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "sb_engine.h"
 #include "sb_host.h"
 
 using namespace sbk;
@@ -973,13 +974,15 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
             return SB_OK;
         }
     }
-    // A batched step streams every instance's J from HBM and there is
-    // parallelism across instances, so the shape is the one with the least
-    // modelled traffic (J read once per replica tile, the B operand once per
-    // row tile), not the fastest solo launch; ties go to the larger tile.
-    // Shapes wider than R rounded up to 64 replicas are skipped (their
-    // padding columns would be multiplied for nothing).  Results do not depend
-    // on the shape.
+    // Candidates: the family's batched shapes, without those wider than R
+    // rounded up to 64 replicas (their padding columns would be multiplied
+    // for nothing).  A batched step streams every instance's J and there is
+    // parallelism across instances, so the shapes are timed on a trial batch
+    // of copies of the instance, not as a solo launch (tune_batch_shape).
+    // Without the autotuner (SB_AUTOTUNE=0) the pick is the shape with the
+    // least modelled traffic (J read once per replica tile, the B operand
+    // once per row tile; ties to the larger tile).  Results do not depend on
+    // the shape.
     const double bj = family == FAM_FP32 || family == FAM_THIN_F32 ? 4.0
                       : family == FAM_SPLIT_BF16 || family == FAM_THIN_BF16 ? 2.0 : 1.0;
     const int bk = plane_kind(family, variant);
@@ -987,9 +990,11 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
     int pick = -1;
     double best_cost = 0.0;
     long best_area = 0;
+    std::vector<int> cands;
     for (int b = 0; b < nb; ++b) {
         const TileShape sh = batch_shape_at(family, b);
         if (b > 0 && sh.nt() > (R + 63) / 64 * 64) continue;
+        cands.push_back(b);
         const double tm = (N + sh.mt() - 1) / sh.mt(), tn = (R + sh.nt() - 1) / sh.nt();
         const double cost = tn * (double)N * N * bj + tm * (double)N * R * bb;
         const long area = (long)sh.mt() * sh.nt();
@@ -998,6 +1003,10 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
             best_cost = cost;
             best_area = area;
         }
+    }
+    if (ctx->autotune && cands.size() > 1) {
+        int rc = tune_batch_shape(ctx, const_cast<sb_problem *>(prob), family, variant, R, cands, pick, &pick);
+        if (rc) return rc;
     }
     ctx->tune[key] = {family, pick};
     *bshape = pick;

@@ -2,6 +2,7 @@
 #include "sb_engine.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <queue>
 
@@ -67,7 +68,7 @@ size_t Engine::sparse_state_bytes(int N, int R)
     return (size_t)N * round_up(R, 64) * sizeof(float) * 3;  // XT[2], YT
 }
 
-int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
+int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out, int force_bshape)
 {
     const int N = prob->N, ld = prob->ld, R = p.replicas;
     const int family = engine_family(p.precision, p.variant, prob, R);
@@ -75,8 +76,13 @@ int Engine::add(sb_problem *prob, const sb_params &p, double c0, int *slot_out)
         set_error("batched solve supports SB_PREC_FP32 and SB_PREC_SPLIT");
         return SB_E_INVALID;
     }
-    int bshape = 0;
-    int rc = family == FAM_SPARSE ? SB_OK : choose_batch_shape(ctx_, prob, family, p.variant, R, &bshape);
+    int bshape = 0, rc = SB_OK;
+    if (family != FAM_SPARSE) {
+        if (force_bshape >= 0 && force_bshape < batch_shape_count(family))
+            bshape = force_bshape;
+        else
+            rc = choose_batch_shape(ctx_, prob, family, p.variant, R, &bshape);
+    }
     if (rc) return rc;
     int idx = -1;
     for (size_t i = 0; i < slots_.size(); ++i)
@@ -446,6 +452,58 @@ int Engine::read(int slot, sb_result *res)
     rc = read_results(ctx_, s.prob, s.X[k & 1], s.Y, R, res, ctx_->stream);
     if (rc) return rc;
     res->c0_used = s.c0;
+    return SB_OK;
+}
+
+// Timed batch-shape choice.  A trial batch holds k copies of the instance
+// (J shared, each copy its own state), enough work items to give the
+// persistent grid about four items per workgroup, as a loaded batch has (at
+// most 128 copies and 1 GiB of state); each candidate shape runs the same trial batch for a replayed
+// chunk of steps after a warm-up chunk, and the fastest is kept.  A shape
+// never changes a result, only its speed.
+int tune_batch_shape(sb_ctx *ctx, sb_problem *prob, int family, int variant, int R, const std::vector<int> &cands,
+                     int model_pick, int *best)
+{
+    const int N = prob->N;
+    sb_params p;
+    sb_default_params(&p);
+    p.variant = variant;
+    p.precision = family == FAM_FP32 || family == FAM_THIN_F32 ? SB_PREC_FP32 : SB_PREC_SPLIT;
+    p.replicas = R;
+    p.steps = 1 << 30;  // never finishes during the trial
+    p.init = SB_INIT_UNIFORM;
+    p.seed = 1;
+    p.dt = 1.0;
+    const TileShape ms = batch_shape_at(family, model_pick);
+    const long items = (long)((N + ms.mt() - 1) / ms.mt()) * (is_thin(family) ? 1 : (R + ms.nt() - 1) / ms.nt());
+    const int grid = batch_grid_size(ctx->device);
+    long k = std::max(1L, (4L * grid + items - 1) / std::max(1L, items));
+    k = std::min(k, 128L);
+    const size_t per_copy = Engine::state_bytes(N, R);
+    k = std::max(1L, std::min(k, (long)((size_t)1 << 30) / (long)std::max<size_t>(per_copy, 1)));
+    const int C = 4;
+    double best_ms = -1.0;
+    *best = model_pick;
+    for (int b : cands) {
+        Engine eng(ctx);
+        int rc = SB_OK;
+        for (long i = 0; i < k && rc == SB_OK; ++i) {
+            int slot = -1;
+            rc = eng.add(prob, p, 0.1, &slot, b);
+        }
+        if (rc == SB_OK) rc = eng.rebuild();
+        if (rc == SB_OK) rc = eng.run_chunk(C);  // warm-up (and graph capture)
+        if (rc == SB_OK) rc = eng.run_chunk(C);
+        if (rc) return rc;
+        const double t = eng.last_chunk_ms();
+        if (ctx->verbose)
+            std::fprintf(stderr, "sb: batch shape family %d shape %d N=%d R=%d x%ld: %.3f ms per %d steps\n", family, b,
+                         N, R, k, t, C);
+        if (best_ms < 0.0 || t < best_ms) {
+            best_ms = t;
+            *best = b;
+        }
+    }
     return SB_OK;
 }
 

@@ -45,7 +45,8 @@ class Engine {
     static size_t state_bytes(int N, int R);
     static size_t sparse_state_bytes(int N, int R);
     // Choose family/shape, allocate state, run init (+ aSB prologue, planes).
-    int add(sb_problem *prob, const sb_params &p, double c0, int *slot);
+    // force_bshape >= 0 skips the choice (the tuner's own trial batches).
+    int add(sb_problem *prob, const sb_params &p, double c0, int *slot, int force_bshape = -1);
     // Free the slot's state.
     void remove(int slot);
     // Rebuild work lists and device tables after add/remove.
@@ -91,5 +92,9 @@ class Engine {
 int engine_family(int precision, int variant, const sb_problem *prob, int R);
 // Autotuned batch shape of a family for (N, R), cached per context.
 int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int variant, int R, int *bshape);
+// Times the candidate batch shapes on trial batches of copies of prob and
+// returns the fastest in *best (choose_batch_shape's timed path).
+int tune_batch_shape(sb_ctx *ctx, sb_problem *prob, int family, int variant, int R, const std::vector<int> &cands,
+                     int model_pick, int *best);
 
 }  // namespace sbh

@@ -1005,8 +1005,15 @@ int choose_batch_shape(sb_ctx *ctx, const sb_problem *prob, int family, int vari
         }
     }
     if (ctx->autotune && cands.size() > 1) {
-        int rc = tune_batch_shape(ctx, const_cast<sb_problem *>(prob), family, variant, R, cands, pick, &pick);
-        if (rc) return rc;
+        // A trial that cannot run (device memory short of the trial batch)
+        // leaves the modelled pick; the failed call's error is cleared so the
+        // next launch check does not report it.
+        int tuned = pick;
+        if (tune_batch_shape(ctx, const_cast<sb_problem *>(prob), family, variant, R, cands, pick, &tuned) ==
+            SB_OK)
+            pick = tuned;
+        else
+            (void)hipGetLastError();
     }
     ctx->tune[key] = {family, pick};
     *bshape = pick;
